@@ -1,0 +1,1291 @@
+// mfg_trace.hip -- HIP kernels of the MI355X test-particle tracer (gfx950).
+//
+// k_trace: one persistent launch traces a whole batch of desorptions. Each
+// lane owns one particle in registers; dead lanes refill from a global work
+// queue with one atomic per wave (ballot + mbcnt). Per event a lane walks a
+// 4-wide BVH (FP32 boxes, traversal stack in LDS laid out [depth][lane] so a
+// wave's pushes hit distinct banks), tests facets in FP64, and applies the
+// event of reference src/Simulation/Particle.cpp:207-396 (desorption, bounce,
+// absorption, transparent pass, teleport, link, leak, volume events).
+// Tallies go straight to HBM with u64 / FP64 atomics; the desorption tallies
+// of lanes that share a source facet are summed across the wave first, and
+// the global counters are kept per lane and flushed once per wave at exit.
+//
+// Random numbers: Philox4x32-10 keyed by the variant seed, counter
+// (particle index, block, 0); a lane draws them in the same order as the
+// CPU restatement, so seeded runs are comparable event by event.
+#include <hip/hip_runtime.h>
+
+#include "mfg_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInv2p32 = 2.3283064365386962890625e-10;
+
+// ------------------------------------------------------------------ RNG
+__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                         uint32_t k1, uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+__device__ __forceinline__ double u2d(uint32_t x) { return ((double)x + 0.5) * kInv2p32; }
+
+struct Rng {
+    uint32_t k0, k1, p0, p1, blk, b0, b1, b2, b3, avail;
+    __device__ void init(uint64_t seed, uint64_t pid) {
+        k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
+        p0 = (uint32_t)pid; p1 = (uint32_t)(pid >> 32);
+        blk = 0; avail = 0;
+    }
+    __device__ __forceinline__ double rnd() {
+        if (avail == 0) {
+            philox10(p0, p1, blk, 0u, k0, k1, b0, b1, b2, b3);
+            ++blk;
+            avail = 4;
+        }
+        uint32_t r = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        --avail;
+        return u2d(r);
+    }
+    __device__ double coin(uint32_t event, uint32_t facet) const {
+        uint32_t o0, o1, o2, o3;
+        philox10(p0, p1, event, 0x80000000u | facet, k0, k1, o0, o1, o2, o3);
+        return u2d(o0);
+    }
+};
+
+// ------------------------------------------------------------------ math helpers
+struct d3 { double x, y, z; };
+__device__ __forceinline__ d3 mk(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ d3 add(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 sub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 scl(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double fdot(d3 a, d3 b) { return fma(a.x, b.x, fma(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double nrm(d3 a) { return sqrt(dot(a, a)); }
+__device__ __forceinline__ d3 unitv(d3 a) { double n = nrm(a); return n > 0 ? scl(1.0 / n, a) : a; }
+__device__ __forceinline__ d3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
+
+__device__ __forceinline__ void atomic_addd(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_addu(void* p, unsigned long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), v);
+}
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ tables
+struct XYRef { const double* x; const double* y; uint32_t n; };
+
+__device__ double interp_y(double x, XYRef t, bool logx, bool logy, bool extrapolate) {
+    uint32_t n = t.n;
+    if (n == 0) return 0.0;
+    if (n == 1) return t.y[0];
+    uint32_t i;
+    if (x <= t.x[0]) {
+        if (!extrapolate) return t.y[0];
+        i = 0;
+    } else if (x >= t.x[n - 1]) {
+        if (!extrapolate) return t.y[n - 1];
+        i = n - 2;
+    } else {
+        uint32_t lo = 0, hi = n - 1;
+        while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (t.x[mid] <= x) lo = mid; else hi = mid; }
+        i = lo;
+    }
+    double x0 = t.x[i], x1 = t.x[i + 1], y0 = t.y[i], y1 = t.y[i + 1];
+    if (logx) { x0 = log10(x0); x1 = log10(x1); x = log10(x); }
+    if (logy) { y0 = log10(y0); y1 = log10(y1); }
+    double y = (x1 == x0) ? y0 : y0 + (x - x0) * (y1 - y0) / (x1 - x0);
+    return logy ? pow(10.0, y) : y;
+}
+
+__device__ double interp_x(double y, XYRef t) {
+    uint32_t n = t.n;
+    if (n == 1) return t.x[0];
+    uint32_t i;
+    if (y < t.y[0]) i = 0;
+    else if (y >= t.y[n - 1]) i = n - 2;
+    else {
+        uint32_t lo = 0, hi = n - 1;
+        while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (t.y[mid] <= y) lo = mid; else hi = mid; }
+        i = lo;
+    }
+    double y0 = t.y[i], y1 = t.y[i + 1], x0 = t.x[i], x1 = t.x[i + 1];
+    if (y1 == y0) return x0;
+    return x0 + (y - y0) * (x1 - x0) / (y1 - y0);
+}
+
+// ------------------------------------------------------------------ geometry
+__device__ __forceinline__ bool in_poly(const double* __restrict__ p, uint32_t n, double u, double v) {
+    bool inside = false;
+    double uj = p[2 * (n - 1)], vj = p[2 * (n - 1) + 1];
+    for (uint32_t i = 0; i < n; ++i) {
+        double ui = p[2 * i], vi = p[2 * i + 1];
+        if ((vi > v) != (vj > v)) {
+            double x = (uj - ui) * (v - vi) / (vj - vi) + ui;
+            if (u < x) inside = !inside;
+        }
+        uj = ui; vj = vi;
+    }
+    return inside;
+}
+
+// Ray/facet test; the same arithmetic, operation for operation, as the CPU
+// restatement (explicit fma dot products, contraction off elsewhere).
+__device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const double* __restrict__ poly, d3 P, d3 D,
+                                           double& t, double& u, double& v) {
+    d3 N = ld3(F->N);
+    double denom = fdot(N, D);
+    uint32_t flags = F->flags;
+    if (denom == 0.0) return false;
+    if (!(flags & FF_2SIDED) && denom > 0.0) return false;
+    d3 O = ld3(F->O);
+    d3 w = sub(O, P);
+    t = fdot(N, w) / denom;
+    if (!(t > 0.0)) return false;
+    d3 X = mk(fma(t, D.x, P.x), fma(t, D.y, P.y), fma(t, D.z, P.z));
+    d3 r = sub(X, O);
+    u = fdot(r, ld3(F->U)) * F->invU2;
+    v = fdot(r, ld3(F->V)) * F->invV2;
+    if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return false;
+    return in_poly(poly + 2 * (size_t)F->poly_off, F->poly_n, u, v);
+}
+
+struct Ray32 { float ox, oy, oz, ix, iy, iz; };
+
+__device__ __forceinline__ float safe_inv(double d) {
+    float f = (float)d;
+    if (fabsf(f) < 1e-30f) f = (f < 0.0f) ? -1e-30f : 1e-30f;
+    return 1.0f / f;
+}
+
+__device__ __forceinline__ Ray32 make_ray32(d3 P, d3 D) {
+    return {(float)P.x, (float)P.y, (float)P.z, safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+}
+
+// Slab test of a node's 4 children against [0, tmax]: hit mask + entry t.
+// Boxes are padded at build time, so the FP32 test never drops a box that
+// holds an FP64 hit.
+__device__ __forceinline__ uint32_t node_hits(const BvhNode* __restrict__ nd, const Ray32& r, float tmax, float tn[4]) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float tx0 = (nd->lox[c] - r.ox) * r.ix, tx1 = (nd->hix[c] - r.ox) * r.ix;
+        float ty0 = (nd->loy[c] - r.oy) * r.iy, ty1 = (nd->hiy[c] - r.oy) * r.iy;
+        float tz0 = (nd->loz[c] - r.oz) * r.iz, tz1 = (nd->hiz[c] - r.oz) * r.iz;
+        float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        tn[c] = a;
+        if (a <= b * 1.000001f) mask |= 1u << c;
+    }
+    return mask & nd->valid;
+}
+
+__device__ __forceinline__ float widen(double t) {
+    if (t > 3.0e38) return 3.0e38f;
+    return (float)t * 1.000004f + 1e-30f;
+}
+
+struct HitRec { int32_t facet; double t, u, v; };
+
+__device__ __forceinline__ double td_opacity(const TraceParams& P, const DVariant* var, int32_t f, double time) {
+    const DPhys& ph = P.phys[f];
+    const DTable tb = P.params[var->param_base + ph.opacity_param];
+    return interp_y(time, XYRef{P.tab_x + tb.off, P.tab_y + tb.off, tb.n}, tb.log_x, tb.log_y, false);
+}
+
+// Nearest hard hit below tmax. With TRANSP, non-opaque facets are hard with
+// probability `opacity` (coin per event and facet); opaque_only ignores
+// opacity (fixed ray batches).
+template <bool TRANSP>
+__device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int32_t skip, double tmax,
+                           int32_t* __restrict__ stk, uint32_t stride, uint32_t stack_size, const Rng* rng,
+                           uint32_t event, const DVariant* var, double time, bool opaque_only) {
+    HitRec h{-1, tmax, 0.0, 0.0};
+    if (root < 0) return h;
+    Ray32 r = make_ray32(Pp, D);
+    float tbf = widen(tmax);
+    uint32_t sp = 0;
+    int32_t node = root;
+    for (;;) {
+        const BvhNode* nd = P.nodes + node;
+        float tn[4];
+        uint32_t m = node_hits(nd, r, tbf, tn);
+        // inner children that survive the (possibly shrunk) tmax, sorted
+        // near-to-far with a fixed network (registers only)
+        constexpr float INF = 3.4e38f;
+        float k0 = INF, k1 = INF, k2 = INF, k3 = INF;
+        int32_t c0 = -1, c1 = -1, c2 = -1, c3 = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (!(m & (1u << c))) continue;
+            int32_t ch = nd->child[c];
+            if (ch >= 0) {
+                if (c == 0) { k0 = tn[0]; c0 = ch; }
+                else if (c == 1) { k1 = tn[1]; c1 = ch; }
+                else if (c == 2) { k2 = tn[2]; c2 = ch; }
+                else { k3 = tn[3]; c3 = ch; }
+                continue;
+            }
+            uint32_t first = (uint32_t)(~ch), cnt = nd->count[c];
+            for (uint32_t k = 0; k < cnt; ++k) {
+                int32_t f = P.leaf_facets[first + k];
+                if (f == skip) continue;
+                const DFacet* F = P.facets + f;
+                double t, u, v;
+                if (!facet_test(F, P.poly, Pp, D, t, u, v)) continue;
+                if (!(t < h.t || (t == h.t && f < h.facet))) continue;
+                bool hard = true;
+                if constexpr (TRANSP) {
+                    uint32_t fl = F->flags;
+                    if (!opaque_only && !(fl & FF_OPAQUE)) {
+                        double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
+                        if (op < 1.0) hard = (op > 0.0) && rng->coin(event, (uint32_t)(f - (int32_t)var->facet_base)) < op;
+                    }
+                }
+                if (hard) {
+                    h.facet = f; h.t = t; h.u = u; h.v = v;
+                    tbf = widen(t);
+                }
+            }
+        }
+        if (k0 > tbf) { k0 = INF; c0 = -1; }
+        if (k1 > tbf) { k1 = INF; c1 = -1; }
+        if (k2 > tbf) { k2 = INF; c2 = -1; }
+        if (k3 > tbf) { k3 = INF; c3 = -1; }
+#define MFG_CSWAP(ka, ca, kb, cb) \
+    if (kb < ka) { float tk = ka; ka = kb; kb = tk; int32_t tc = ca; ca = cb; cb = tc; }
+        MFG_CSWAP(k0, c0, k1, c1)
+        MFG_CSWAP(k2, c2, k3, c3)
+        MFG_CSWAP(k0, c0, k2, c2)
+        MFG_CSWAP(k1, c1, k3, c3)
+        MFG_CSWAP(k1, c1, k2, c2)
+#undef MFG_CSWAP
+        if (c0 >= 0) {
+            if (c3 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c3; ++sp; }
+            if (c2 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c2; ++sp; }
+            if (c1 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c1; ++sp; }
+            node = c0;
+            continue;
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stk[(size_t)sp * stride];
+    }
+    return h;
+}
+
+// ------------------------------------------------------------------ tracer
+template <uint32_t FEAT>
+struct Tracer {
+    const TraceParams& P;
+    int32_t* stk;
+    uint32_t stride;
+    // particle state (reference Particle.h:95-122)
+    d3 pos, dir;
+    double time, vel, ori, gen_time, dist, decay_t, scatter_path, hu, hv;
+    int32_t structure, last_hit, tele_from;
+    uint32_t nb_bounces, nb_events;
+    uint64_t pid;
+    Rng rng;
+    const DVariant* var;
+    // lane accumulators (flushed once per wave)
+    uint32_t acc_hits, acc_des;
+    double acc_hit_equiv, acc_abs_equiv, acc_dist_total, acc_dist_full;
+    uint32_t acc_leak, acc_decay;
+    uint32_t tr_n;
+    uint32_t* adj_cnt;
+
+    __device__ Tracer(const TraceParams& p, int32_t* s, uint32_t st) : P(p), stk(s), stride(st) {
+        acc_hits = acc_des = 0;
+        acc_hit_equiv = acc_abs_equiv = acc_dist_total = acc_dist_full = 0.0;
+        acc_leak = acc_decay = 0;
+        tr_n = 0;
+        adj_cnt = nullptr;
+        var = nullptr;
+    }
+
+    __device__ int32_t local(int32_t f) const { return f - (int32_t)var->facet_base; }
+
+    __device__ void trace(int32_t f, int32_t kind) {
+        if constexpr ((FEAT & FEAT_TRACE) != 0) {
+            if (pid < P.trace_n && tr_n < P.trace_k) {
+                size_t at = (size_t)pid * P.trace_k + tr_n;
+                P.trace_facet[at] = f < 0 ? f : local(f);
+                P.trace_kind[at] = kind;
+                ++tr_n;
+            }
+        }
+        (void)f; (void)kind;
+    }
+    __device__ int moment(double t) const {
+        if constexpr ((FEAT & FEAT_MOMENTS) != 0) {
+            uint32_t n = var->n_moments;
+            if (n == 0) return -1;
+            const double* S = P.mom_start + var->mom_base;
+            uint32_t lo = 0, hi = n;  // first start >= t
+            while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (S[mid] < t) lo = mid + 1; else hi = mid; }
+            uint32_t lb = lo;
+            if (lb != 0) --lb;
+            if (S[lb] <= t && t < P.mom_end[var->mom_base + lb]) return (int)lb + 1;
+        }
+        (void)t;
+        return -1;
+    }
+    __device__ double param_at(int32_t id, double t, double c) const {
+        if constexpr ((FEAT & FEAT_TDPARAM) != 0) {
+            if (id >= 0) {
+                const DTable tb = P.params[var->param_base + id];
+                return interp_y(t, XYRef{P.tab_x + tb.off, P.tab_y + tb.off, tb.n}, tb.log_x, tb.log_y, false);
+            }
+        }
+        (void)id; (void)t;
+        return c;
+    }
+    __device__ double sticking(const DPhys& ph) const {
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            if (ph.adj_group >= 0) return P.adj_sticking[ph.adj_group];
+        }
+        return param_at(ph.sticking_param, time, ph.sticking);
+    }
+    __device__ double temperature(const DPhys& ph) const { return param_at(ph.temperature_param, time, ph.temperature); }
+
+    __device__ mfg_facet_counters* fc(const DPhys& ph, int slot) const {
+        return P.fc + var->fc_base + (size_t)slot * var->n_facets + (uint32_t)ph.local_id;
+    }
+
+    // IncreaseFacetCounter (Particle.cpp:1193-1230), slot 0 and slot m.
+    __device__ void counter(const DPhys& ph, int m, uint32_t hit, uint32_t des, uint32_t abs_, double s1v, double svo,
+                            d3 imp, d3 imp2, d3 impm) {
+        double he = (double)hit * ori;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int slot = k == 0 ? 0 : m;
+            if (k == 1 && m <= 0) break;
+            mfg_facet_counters* c = fc(ph, slot);
+            if (hit) { atomic_addu(&c->nb_mc_hit, hit); atomic_addd(&c->nb_hit_equiv, he); }
+            if (des) atomic_addu(&c->nb_desorbed, des);
+            if (abs_) atomic_addd(&c->nb_abs_equiv, (double)abs_ * ori);
+            if (s1v != 0.0) atomic_addd(&c->sum_1_per_ort_velocity, ori * s1v);
+            if (svo != 0.0) atomic_addd(&c->sum_v_ort, ori * svo);
+            if (hit || des) atomic_addd(&c->sum_1_per_velocity, (he + (double)des) / vel);
+            if constexpr ((FEAT & FEAT_FORCE) != 0) {
+                if (var->enable_force) {
+                    double* fo = P.force + var->force_base + ((size_t)slot * var->n_facets + (uint32_t)ph.local_id) * 9;
+                    atomic_addd(fo + 0, ori * imp.x); atomic_addd(fo + 1, ori * imp.y); atomic_addd(fo + 2, ori * imp.z);
+                    atomic_addd(fo + 3, ori * imp2.x); atomic_addd(fo + 4, ori * imp2.y); atomic_addd(fo + 5, ori * imp2.z);
+                    atomic_addd(fo + 6, ori * impm.x); atomic_addd(fo + 7, ori * impm.y); atomic_addd(fo + 8, ori * impm.z);
+                }
+            }
+        }
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            if (hit && ph.adj_target >= 0) adjoint_score(ph, he, false);
+        }
+        (void)imp; (void)imp2; (void)impm;
+    }
+    // Path score of a hit-equivalent x on a target:
+    // d x / d s_g = x * (-n_g / (1 - s_g)) (+ x / s_t for the absorbed part).
+    __device__ void adjoint_score(const DPhys& ph, double x, bool absorbed_part) {
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            int t = ph.adj_target;
+            atomic_addd(P.adj_j0 + t, x);
+            uint32_t G = P.adj_groups;
+            for (uint32_t g = 0; g < G; ++g) {
+                double s = P.adj_sticking[g];
+                double d = -x * (double)adj_cnt[g] / (1.0 - s);
+                if (absorbed_part && ph.adj_group == (int)g) d += x / s;
+                if (d != 0.0) atomic_addd(P.adj_jac + (size_t)t * G + g, d);
+            }
+        }
+        (void)ph; (void)x; (void)absorbed_part;
+    }
+    __device__ void counter_force(const DPhys& ph, int m, uint32_t hit, uint32_t des, uint32_t abs_, double s1v,
+                                  double svo, double sign) {
+        d3 z = mk(0, 0, 0), v2 = z, mom = z, vv = z;
+        if constexpr ((FEAT & FEAT_FORCE) != 0) {
+            if (var->enable_force) {
+                vv = scl(sign * vel, dir);
+                v2 = mk(vv.x * vv.x, vv.y * vv.y, vv.z * vv.z);
+                mom = cross(sub(pos, ld3(var->torque)), vv);
+            }
+        }
+        (void)sign;
+        counter(ph, m, hit, des, abs_, s1v, svo, vv, v2, mom);
+    }
+    __device__ bool cell(const DPhys& ph, uint32_t& add) const {
+        int tu = (int)(hu * ph.tex_wp), tv = (int)(hv * ph.tex_hp);
+        tu = tu > ph.tex_w - 1 ? ph.tex_w - 1 : tu;
+        tv = tv > ph.tex_h - 1 ? ph.tex_h - 1 : tv;
+        if (tu < 0 || tv < 0) return false;
+        add = (uint32_t)tu + (uint32_t)tv * (uint32_t)ph.tex_w;
+        return true;
+    }
+    __device__ double ort(const DFacet* F) const { return fabs(dot(dir, ld3(F->N))); }
+    // RecordHitOnTexture (Particle.cpp:977-1004)
+    __device__ void texture(const DFacet* F, const DPhys& ph, int m, bool count, double vf, double of) {
+        uint32_t add;
+        if (!cell(ph, add)) return;
+        double ov = var->c_factor * vel * ort(F);
+        double inc = P.tex_inc[var->tex_base + (size_t)ph.tex_off + add];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int slot = k == 0 ? 0 : m;
+            if (k == 1 && m <= 0) break;
+            mfg_texture_cell* c = P.tex + var->tex_base + (size_t)slot * var->n_tex + (size_t)ph.tex_off + add;
+            if (count) atomic_addd(&c->count_equiv, ori);
+            atomic_addd(&c->sum_1_per_ort_velocity, ori * vf / ov);
+            atomic_addd(&c->sum_v_ort_per_area, ori * of * ov * inc);
+        }
+    }
+    // RecordDirectionVector (Particle.cpp:1006-1023)
+    __device__ void direction(const DPhys& ph, int m) {
+        if (ph.dir_off < 0) return;
+        uint32_t add;
+        if (!cell(ph, add)) return;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int slot = k == 0 ? 0 : m;
+            if (k == 1 && m <= 0) break;
+            mfg_direction_cell* c = P.dir + var->dir_base + (size_t)slot * var->n_dir + (size_t)ph.dir_off + add;
+            atomic_addd(&c->dir[0], ori * dir.x * vel);
+            atomic_addd(&c->dir[1], ori * dir.y * vel);
+            atomic_addd(&c->dir[2], ori * dir.z * vel);
+            atomic_addu(&c->count, 1ull);
+        }
+    }
+    // ProfileFacet (Particle.cpp:1025-1085)
+    __device__ void profile(const DFacet* F, const DPhys& ph, int m, bool count, double vf, double of) {
+        if (ph.prof_off < 0) return;
+        int pt = ph.profile_type;
+        double a = ort(F);
+        mfg_profile_slice* base = P.prof + var->prof_base + (size_t)ph.prof_off;
+        if (count && pt == MFG_PROFILE_ANGULAR) {
+            double th = acos(fmin(1.0, a));
+            long p = (long)(th / (kPi / 2) * (double)MFG_PROFILE_SIZE);
+            p = p < 0 ? 0 : (p > MFG_PROFILE_SIZE - 1 ? MFG_PROFILE_SIZE - 1 : p);
+            for (int k = 0; k < 2; ++k) {
+                int slot = k == 0 ? 0 : m;
+                if (k == 1 && m <= 0) break;
+                atomic_addd(&base[(size_t)slot * var->n_prof + p].count_equiv, ori);
+            }
+        } else if (pt == MFG_PROFILE_U || pt == MFG_PROFILE_V) {
+            double c = pt == MFG_PROFILE_U ? hu : hv;
+            if (!(c >= 0.0)) return;
+            double x = c * (double)MFG_PROFILE_SIZE;
+            if (!(x < (double)MFG_PROFILE_SIZE)) return;
+            size_t p = (size_t)x;
+            double ov = vel * a;
+            for (int k = 0; k < 2; ++k) {
+                int slot = k == 0 ? 0 : m;
+                if (k == 1 && m <= 0) break;
+                mfg_profile_slice* s = &base[(size_t)slot * var->n_prof + p];
+                if (count) atomic_addd(&s->count_equiv, ori);
+                atomic_addd(&s->sum_1_per_ort_velocity, ori * vf / ov);
+                atomic_addd(&s->sum_v_ort, ori * of * var->c_factor * ov);
+            }
+        } else if (count && (pt == MFG_PROFILE_VELOCITY || pt == MFG_PROFILE_ORT_VELOCITY || pt == MFG_PROFILE_TAN_VELOCITY)) {
+            double d = pt == MFG_PROFILE_VELOCITY ? 1.0 : (pt == MFG_PROFILE_ORT_VELOCITY ? a : sqrt(fmax(0.0, 1 - a * a)));
+            double x = d * vel / ph.max_speed * (double)MFG_PROFILE_SIZE;
+            if (!(x >= 0.0) || !(x < (double)MFG_PROFILE_SIZE)) return;
+            size_t p = (size_t)x;
+            for (int k = 0; k < 2; ++k) {
+                int slot = k == 0 ? 0 : m;
+                if (k == 1 && m <= 0) break;
+                atomic_addd(&base[(size_t)slot * var->n_prof + p].count_equiv, ori);
+            }
+        }
+    }
+    __device__ void polar(const DFacet* F, const DPhys& ph, double& th, double& phv) const {
+        double lu = dot(dir, ld3(ph.nU)), lv = dot(dir, ld3(ph.nV)), ln = dot(dir, ld3(F->N));
+        th = acos(fmax(-1.0, fmin(1.0, ln)));
+        phv = atan2(lv, lu);
+    }
+    // RecordAngleMap (Particle.cpp:1104-1136)
+    __device__ void angle_map(const DFacet* F, const DPhys& ph) {
+        if (!(ph.flags & FF_ANGLEMAP) || ph.am_off < 0) return;
+        double th, phv;
+        polar(F, ph, th, phv);
+        if (th > kPi / 2.0) th = fabs(kPi - th);
+        long ti;
+        if (th < ph.am_theta_limit) {
+            if (ph.am_lo == 0) return;
+            ti = (long)(th / ph.am_theta_limit * (double)ph.am_lo);
+        } else {
+            if (ph.am_hi == 0) return;
+            ti = (long)ph.am_lo + (long)((th - ph.am_theta_limit) / (kPi / 2.0 - ph.am_theta_limit) * (double)ph.am_hi);
+        }
+        long nt = (long)(ph.am_lo + ph.am_hi);
+        ti = ti < 0 ? 0 : (ti > nt - 1 ? nt - 1 : ti);
+        long pi_ = (long)((phv + 3.1415926) / (2.0 * kPi) * (double)ph.am_phi);
+        pi_ = pi_ < 0 ? 0 : (pi_ > (long)ph.am_phi - 1 ? (long)ph.am_phi - 1 : pi_);
+        atomic_addu(P.am + var->am_base + (size_t)ph.am_off + (size_t)ti * ph.am_phi + pi_, 1ull);
+    }
+    // LogHit (Particle.cpp:1087-1102)
+    __device__ void log_hit(const DFacet* F, const DPhys& ph, int32_t f) {
+        if constexpr ((FEAT & FEAT_LOG) != 0) {
+            if (var->log_facet == f) {
+                unsigned long long at = atomicAdd(P.log_count, 1ull);
+                if (at < var->log_limit) {
+                    mfg_log_entry e;
+                    e.u = hu; e.v = hv;
+                    polar(F, ph, e.hit_theta, e.hit_phi);
+                    e.ori_ratio = ori; e.particle_decay_moment = decay_t; e.time = time; e.velocity = vel;
+                    P.log[at] = e;
+                }
+            }
+        }
+        (void)F; (void)ph; (void)f;
+    }
+    __device__ void hist_add(double* g, uint32_t nb, uint32_t nd, uint32_t nt, uint32_t bbin, double dbin, double tbin) {
+        uint32_t off = 0;
+        if (nb) { uint32_t b = nb_bounces / bbin; atomic_addd(g + off + (b < nb - 1 ? b : nb - 1), ori); off += nb; }
+        if (nd) {
+            double x = dist / dbin;
+            uint32_t b = x < (double)(nd - 1) ? (uint32_t)x : nd - 1;
+            atomic_addd(g + off + b, ori); off += nd;
+        }
+        if (nt) {
+            double x = (time - gen_time) / tbin;
+            uint32_t b = x < (double)(nt - 1) ? (uint32_t)x : nt - 1;
+            atomic_addd(g + off + b, ori);
+        }
+    }
+    // RecordHistograms (Particle.cpp:926-975), called on absorption
+    __device__ void histograms(const DPhys& ph, int m) {
+        if constexpr ((FEAT & FEAT_HIST) != 0) {
+            for (int k = 0; k < 2; ++k) {
+                int slot = k == 0 ? 0 : m;
+                if (k == 1 && m <= 0) break;
+                if (var->n_gh)
+                    hist_add(P.gh + var->gh_base + (size_t)slot * var->n_gh, var->gh_nb_bounce, var->gh_nb_dist,
+                             var->gh_nb_time, var->gh_bounce_bin, var->gh_dist_bin, var->gh_time_bin);
+                if (ph.fh_off >= 0)
+                    hist_add(P.fh + var->fh_base + (size_t)slot * var->n_fh + ph.fh_off, ph.fh_nb_bounce, ph.fh_nb_dist,
+                             ph.fh_nb_time, ph.fh_bounce_bin, ph.fh_dist_bin, ph.fh_time_bin);
+            }
+        }
+        (void)ph; (void)m;
+    }
+    __device__ d3 polar_to_cart(const DFacet* F, const DPhys& ph, double cos_t, double sin_t, double phv, bool reverse) const {
+        double s, c;
+        sincos(phv, &s, &c);
+        double n = reverse ? -cos_t : cos_t;
+        return add(add(scl(sin_t * c, ld3(ph.nU)), scl(sin_t * s, ld3(ph.nV))), scl(n, ld3(F->N)));
+    }
+    // TreatMovingFacet (Physics.cpp:22-39)
+    __device__ void moving(d3& d, double& v) const {
+        if constexpr ((FEAT & FEAT_MOVING) != 0) {
+            d3 a = mk(0, 0, 0);
+            if (var->motion_type == 1) a = ld3(var->motion2);
+            else if (var->motion_type == 2) a = cross(ld3(var->motion2), scl(0.01, sub(pos, ld3(var->motion1))));
+            d3 nv = add(scl(v, d), a);
+            v = nrm(nv);
+            d = unitv(nv);
+        }
+        (void)d; (void)v;
+    }
+    __device__ double maxwell_speed(double sq) {
+        XYRef t{P.tab_x + var->maxwell_base, P.tab_y + var->maxwell_base, 100};
+        return interp_x(rng.rnd(), t) * sq;
+    }
+    __device__ d3 emit(const DFacet* F, const DPhys& ph, int type, double n_exp, bool reverse) {
+        if (type == MFG_DES_UNIFORM) {
+            double c = rng.rnd(); double phv = rng.rnd() * 2.0 * kPi;
+            return polar_to_cart(F, ph, c, sqrt(1 - c * c), phv, reverse);
+        }
+        if (type == MFG_DES_COSINE_N) {
+            double c = pow(rng.rnd(), 1.0 / (n_exp + 1.0)); double phv = rng.rnd() * 2.0 * kPi;
+            return polar_to_cart(F, ph, c, sqrt(1 - c * c), phv, reverse);
+        }
+        double a = rng.rnd(); double phv = rng.rnd() * 2.0 * kPi;
+        return polar_to_cart(F, ph, sqrt(a), sqrt(1 - a), phv, reverse);
+    }
+
+    // StartFromSource (Particle.cpp:407-685) without the source-facet
+    // counter, which the caller adds (summed across the wave).
+    __device__ int32_t start(uint64_t local_pid) {
+        pid = local_pid;
+        rng.init(var->seed, pid);
+        nb_events = 0;
+        tr_n = 0;
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) { for (uint32_t g = 0; g < P.adj_groups; ++g) adj_cnt[g] = 0; }
+        double r = rng.rnd() * var->total_des;
+        const double* cum = P.src_cum + var->src_base;
+        uint32_t n = var->src_n;
+        uint32_t lo = 0, hi = n;  // first cum > r
+        while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (cum[mid] <= r) lo = mid + 1; else hi = mid; }
+        uint32_t k = lo < n ? lo : n - 1;
+        int32_t src = P.src_facet[var->src_base + k];
+        const DFacet* F = P.facets + src;
+        const DPhys& ph = P.phys[src];
+        bool reverse = false;
+        if (F->flags & FF_2SIDED) reverse = rng.rnd() > 0.5;
+        long map_w = -1, map_h = -1;
+        if constexpr ((FEAT & FEAT_OGMAP) != 0) {
+            if (ph.flags & FF_OGMAP) {
+                double base = k ? cum[k - 1] : 0.0;
+                double rem = (r - base) / var->latest * (1.38E-23 * ph.temperature);
+                const double* oc = P.og_cdf + ph.og_off;
+                uint32_t nc = ph.og_w * ph.og_h, a = 0, b = nc;
+                while (a < b) { uint32_t mid = (a + b) / 2; if (oc[mid] <= rem) a = mid + 1; else b = mid; }
+                uint32_t cellv = a < nc ? a : nc - 1;
+                map_h = (long)(cellv / ph.og_w);
+                map_w = (long)cellv - map_h * (long)ph.og_w;
+            }
+        }
+        last_hit = src;
+        double rt = rng.rnd();
+        time = rt * var->latest;
+        if constexpr ((FEAT & FEAT_TDPARAM) != 0) {
+            if (ph.id_index >= 0) {
+                const DTable tb = P.ids[var->id_base + ph.id_index];
+                XYRef t{P.tab_x + tb.off, P.tab_y + tb.off, tb.n};
+                time = interp_x(rt * t.y[t.n - 1], t);
+            }
+        }
+        gen_time = time;
+        if (var->use_maxwell) vel = maxwell_speed(ph.sqrt_temp);
+        else vel = 145.469 * sqrt(ph.temperature / var->gas_mass);
+        ori = 1.0;
+        decay_t = 1e100;
+        scatter_path = 1e100;
+        if constexpr ((FEAT & FEAT_VOLUME) != 0) {
+            if (var->enable_decay) decay_t = time + (1.44269 * var->half_life) * -log(rng.rnd());
+            if (var->scatter_enabled) scatter_path = var->scatter_mfp * -log(rng.rnd());
+        }
+        nb_bounces = 0;
+        dist = 0.0;
+        bool found = false;
+        const double* poly = P.poly + 2 * (size_t)F->poly_off;
+        uint32_t pn = F->poly_n;
+        d3 O = ld3(F->O), U = ld3(F->U), V = ld3(F->V);
+        for (int tr = 0; tr < 1000 && !found; ++tr) {
+            double u, v;
+            if ((FEAT & FEAT_OGMAP) != 0 && map_w >= 0) {
+                double a = rng.rnd(), b = rng.rnd();
+                double wp = ph.og_wp, hp = ph.og_hp;
+                u = (map_w < (long)ph.og_w - 1) ? ((double)map_w + a) / wp : ((double)map_w + a * (wp - ((double)ph.og_w - 1.0))) / wp;
+                v = (map_h < (long)ph.og_h - 1) ? ((double)map_h + b) / hp : ((double)map_h + b * (hp - ((double)ph.og_h - 1.0))) / hp;
+            } else {
+                u = rng.rnd();
+                v = rng.rnd();
+            }
+            if (in_poly(poly, pn, u, v)) {
+                pos = add(add(O, scl(u, U)), scl(v, V));
+                hu = u; hv = v;
+                found = true;
+            }
+        }
+        if (!found) {
+            if ((FEAT & FEAT_OGMAP) != 0 && map_w >= 0) {
+                hu = ((double)map_w + 0.5) / ph.og_wp;
+                hv = ((double)map_h + 0.5) / ph.og_hp;
+                pos = add(add(O, scl(hu, U)), scl(hv, V));
+            } else {
+                hu = 0.5; hv = 0.5;
+                pos = ld3(ph.centre);
+            }
+        }
+        dir = emit(F, ph, ph.desorb_type, ph.des_exp, reverse);
+        structure = ph.super_idx;
+        tele_from = -1;
+        if constexpr ((FEAT & FEAT_MOVING) != 0) {
+            if ((ph.flags & FF_MOVING) && var->motion_type) moving(dir, vel);
+        }
+        return src;
+    }
+
+    // Records of a desorption other than the facet counter of slot 0.
+    __device__ void desorption_records(int32_t src, int m, double ov) {
+        const DFacet* F = P.facets + src;
+        const DPhys& ph = P.phys[src];
+        if (m > 0 || ((FEAT & FEAT_FORCE) != 0 && var->enable_force)) {
+            // moment slot (and force terms) per lane
+            d3 z = mk(0, 0, 0), v2 = z, mom = z, vv = z;
+            if constexpr ((FEAT & FEAT_FORCE) != 0) {
+                if (var->enable_force) {
+                    vv = scl(vel, dir);
+                    v2 = mk(vv.x * vv.x, vv.y * vv.y, vv.z * vv.z);
+                    mom = cross(sub(pos, ld3(var->torque)), vv);
+                    double* fo = P.force + var->force_base + (size_t)(uint32_t)ph.local_id * 9;
+                    atomic_addd(fo + 0, vv.x); atomic_addd(fo + 1, vv.y); atomic_addd(fo + 2, vv.z);
+                    atomic_addd(fo + 3, v2.x); atomic_addd(fo + 4, v2.y); atomic_addd(fo + 5, v2.z);
+                    atomic_addd(fo + 6, mom.x); atomic_addd(fo + 7, mom.y); atomic_addd(fo + 8, mom.z);
+                }
+            }
+            if (m > 0) {
+                mfg_facet_counters* c = fc(ph, m);
+                atomic_addu(&c->nb_desorbed, 1ull);
+                atomic_addd(&c->sum_1_per_ort_velocity, 2.0 / ov);
+                atomic_addd(&c->sum_v_ort, var->c_factor * ov);
+                atomic_addd(&c->sum_1_per_velocity, 1.0 / vel);
+                if constexpr ((FEAT & FEAT_FORCE) != 0) {
+                    if (var->enable_force) {
+                        double* fo = P.force + var->force_base + ((size_t)m * var->n_facets + (uint32_t)ph.local_id) * 9;
+                        atomic_addd(fo + 0, vv.x); atomic_addd(fo + 1, vv.y); atomic_addd(fo + 2, vv.z);
+                        atomic_addd(fo + 3, v2.x); atomic_addd(fo + 4, v2.y); atomic_addd(fo + 5, v2.z);
+                        atomic_addd(fo + 6, mom.x); atomic_addd(fo + 7, mom.y); atomic_addd(fo + 8, mom.z);
+                    }
+                }
+            }
+        }
+        profile(F, ph, m, false, 2.0, 1.0);
+        log_hit(F, ph, src);
+        if ((ph.flags & FF_COUNT_DES) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 1.0);
+        trace(src, 0);
+    }
+
+    // RegisterTransparentPass (Particle.cpp:1232-1257)
+    __device__ void transparent_pass(int32_t f, double tp) {
+        const DFacet* F = P.facets + f;
+        const DPhys& ph = P.phys[f];
+        double df = ort(F);
+        int m = moment(time + tp / 100.0 / vel);
+        d3 z = mk(0, 0, 0);
+        counter(ph, m, 1, 0, 0, 2.0 / (vel * df), 2.0 * var->c_factor * vel * df, z, z, z);
+        if ((ph.flags & FF_COUNT_TRANS) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 2.0);
+        if (ph.flags & FF_COUNT_DIR) direction(ph, m);
+        log_hit(F, ph, f);
+        profile(F, ph, m, true, 2.0, 2.0);
+        angle_map(F, ph);
+        trace(f, 3);
+    }
+    // RecordAbsorb (Particle.cpp:888-924)
+    __device__ void record_absorb(int32_t f) {
+        const DFacet* F = P.facets + f;
+        const DPhys& ph = P.phys[f];
+        ++acc_hits; acc_hit_equiv += ori; acc_abs_equiv += ori;
+        int m = moment(time);
+        histograms(ph, m);
+        double ov = vel * ort(F);
+        bool scored = false;
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            if (ph.adj_target >= 0) {
+                // the absorbed part carries its own d/ds_t term
+                for (int k = 0; k < 2; ++k) {
+                    int slot = k == 0 ? 0 : m;
+                    if (k == 1 && m <= 0) break;
+                    mfg_facet_counters* c = fc(ph, slot);
+                    atomic_addu(&c->nb_mc_hit, 1ull);
+                    atomic_addd(&c->nb_hit_equiv, ori);
+                    atomic_addd(&c->nb_abs_equiv, ori);
+                    atomic_addd(&c->sum_1_per_ort_velocity, ori * 2.0 / ov);
+                    atomic_addd(&c->sum_v_ort, ori * var->c_factor * ov);
+                    atomic_addd(&c->sum_1_per_velocity, ori / vel);
+                }
+                adjoint_score(ph, ori, true);
+                scored = true;
+            }
+        }
+        if (!scored) counter_force(ph, m, 1, 0, 1, 2.0 / ov, var->c_factor * ov, 1.0);
+        log_hit(F, ph, f);
+        profile(F, ph, m, true, 2.0, 1.0);
+        angle_map(F, ph);
+        if ((ph.flags & FF_COUNT_ABS) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 1.0);
+        if (ph.flags & FF_COUNT_DIR) direction(ph, m);
+    }
+    // UpdateVelocity (Particle.cpp:1138-1163)
+    __device__ void update_velocity(const DPhys& ph) {
+        double T = temperature(ph);
+        double sq = ph.temperature_param == -1 ? ph.sqrt_temp : sqrt(T);
+        if (ph.flags & FF_FULL_ACC) {
+            vel = var->use_maxwell ? maxwell_speed(sq) : 145.469 * sq / var->gas_mass;
+        } else {
+            double o2 = vel * vel, n2;
+            if (var->use_maxwell) { double vv = maxwell_speed(sq); n2 = vv * vv; }
+            else n2 = 29369.939 * (T / var->gas_mass);
+            vel = sqrt(o2 + (n2 - o2) * ph.accomodation);
+        }
+    }
+    // PerformBounce (Particle.cpp:691-838)
+    __device__ void bounce(int32_t f) {
+        const DFacet* F = P.facets + f;
+        const DPhys& ph = P.phys[f];
+        ++acc_hits; acc_hit_equiv += ori;
+        if constexpr ((FEAT & FEAT_TELEPORT) != 0) {
+            if (ph.super_dest) {
+                int m = moment(time);
+                d3 z = mk(0, 0, 0);
+                counter(ph, m, 1, 0, 0, 0.0, 0.0, z, z, z);
+                structure = ph.super_dest - 1;
+                if constexpr ((FEAT & FEAT_MOVING) != 0) { if (ph.flags & FF_MOVING) moving(dir, vel); }
+                log_hit(F, ph, f);
+                profile(F, ph, m, true, 2.0, 2.0);
+                angle_map(F, ph);
+                if ((ph.flags & FF_COUNT_TRANS) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 2.0);
+                if (ph.flags & FF_COUNT_DIR) direction(ph, m);
+                trace(f, 5);
+                return;
+            }
+        }
+        d3 N = ld3(F->N);
+        double dn = dot(dir, N);
+        bool revert = (F->flags & FF_2SIDED) && dn > 0.0;
+        double ov_in = vel * fabs(dn);
+        double v_in = vel;
+        int m = moment(time);
+        bool simple = (FEAT & (FEAT_FORCE | FEAT_ADJOINT | FEAT_SOJOURN)) == 0;
+        if (!simple) counter_force(ph, m, 1, 0, 0, 1.0 / ov_in, var->c_factor * ov_in, 1.0);
+        ++nb_bounces;
+        if ((ph.flags & FF_COUNT_REFL) && ph.tex_off >= 0) texture(F, ph, m, true, 1.0, 1.0);
+        if (ph.flags & FF_COUNT_DIR) direction(ph, m);
+        log_hit(F, ph, f);
+        profile(F, ph, m, true, 1.0, 1.0);
+        angle_map(F, ph);
+        update_velocity(ph);
+        int m_out = m;
+        if constexpr ((FEAT & FEAT_SOJOURN) != 0) {
+            if (ph.flags & FF_SOJOURN) {
+                double A = exp(-ph.sojourn_e / (8.31 * temperature(ph)));
+                time += -log(rng.rnd()) / (A * ph.sojourn_freq);
+                m_out = moment(time);
+            }
+        }
+        if (ph.flags & FF_DIFFUSE) {
+            double a = rng.rnd(); double phv = rng.rnd() * 2.0 * kPi;
+            dir = polar_to_cart(F, ph, sqrt(a), sqrt(1 - a), phv, revert);
+        } else {
+            double rt = rng.rnd();
+            if (rt < ph.diffuse) {
+                double a = rng.rnd(); double phv = rng.rnd() * 2.0 * kPi;
+                dir = polar_to_cart(F, ph, sqrt(a), sqrt(1 - a), phv, revert);
+            } else if (rt < ph.diffuse + ph.specular) {
+                dir = sub(dir, scl(2.0 * dot(dir, N), N));
+            } else {
+                double c = pow(rng.rnd(), 1.0 / (ph.cos_exp + 1.0)); double phv = rng.rnd() * 2.0 * kPi;
+                dir = polar_to_cart(F, ph, c, sqrt(1 - c * c), phv, revert);
+            }
+        }
+        if constexpr ((FEAT & FEAT_MOVING) != 0) {
+            if ((ph.flags & FF_MOVING) && var->motion_type) moving(dir, vel);
+        }
+        double ov_out = vel * ort(F);
+        if (simple) {
+            // incoming and outgoing counters of the same slot in one pass
+            for (int k = 0; k < 2; ++k) {
+                int slot = k == 0 ? 0 : m;
+                if (k == 1 && m <= 0) break;
+                mfg_facet_counters* c = fc(ph, slot);
+                atomic_addu(&c->nb_mc_hit, 1ull);
+                atomic_addd(&c->nb_hit_equiv, ori);
+                atomic_addd(&c->sum_1_per_ort_velocity, ori * (1.0 / ov_in + 1.0 / ov_out));
+                atomic_addd(&c->sum_v_ort, ori * var->c_factor * (ov_in + ov_out));
+                atomic_addd(&c->sum_1_per_velocity, ori / v_in);
+            }
+        } else {
+            counter_force(ph, m_out, 0, 0, 0, 1.0 / ov_out, var->c_factor * ov_out, -1.0);
+        }
+        if ((ph.flags & FF_COUNT_REFL) && ph.tex_off >= 0) texture(F, ph, m_out, false, 1.0, 1.0);
+        profile(F, ph, m_out, false, 1.0, 1.0);
+        last_hit = f;
+        trace(f, 1);
+    }
+    // PerformTeleport (Particle.cpp:76-183)
+    __device__ void teleport(int32_t f) {
+        const DFacet* F = P.facets + f;
+        const DPhys& ph = P.phys[f];
+        trace(f, 4);
+        int32_t dest = ph.teleport_dest == -1 ? tele_from : ph.teleport_dest - 1;
+        if (dest < 0 || dest >= (int32_t)var->n_facets) { last_hit = f; return; }
+        int32_t dg = dest + (int32_t)var->facet_base;
+        const DFacet* D = P.facets + dg;
+        const DPhys& dph = P.phys[dg];
+        if (dph.super_idx >= 0) structure = dph.super_idx;
+        tele_from = local(f);
+        int m = moment(time);
+        if ((ph.flags & FF_COUNT_TRANS) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 2.0);
+        if (ph.flags & FF_COUNT_DIR) direction(ph, m);
+        profile(F, ph, m, true, 2.0, 2.0);
+        log_hit(F, ph, f);
+        angle_map(F, ph);
+        double lu = dot(dir, ld3(ph.nU)), lv = dot(dir, ld3(ph.nV)), ln = dot(dir, ld3(F->N));
+        dir = add(add(scl(lu, ld3(dph.nU)), scl(lv, ld3(dph.nV))), scl(ln, ld3(D->N)));
+        double u = hu, v = hv;
+        d3 O = ld3(D->O), U = ld3(D->U), V = ld3(D->V);
+        pos = add(add(O, scl(u, U)), scl(v, V));
+        const double* poly = P.poly + 2 * (size_t)D->poly_off;
+        if (!in_poly(poly, D->poly_n, u, v)) {
+            for (int tr = 0; tr < 1000; ++tr) {
+                u = rng.rnd(); v = rng.rnd();
+                if (in_poly(poly, D->poly_n, u, v)) { pos = add(add(O, scl(u, U)), scl(v, V)); break; }
+            }
+        }
+        last_hit = dg;
+        double ov = vel * ort(F);
+        d3 z = mk(0, 0, 0);
+        counter(ph, m, 1, 0, 0, 2.0 / ov, 2.0 * var->c_factor * ov, z, z, z);
+    }
+    // PerformScatter (Particle.cpp:840-886, Physics.cpp:41-63)
+    __device__ bool scatter() {
+        ++acc_hits; acc_hit_equiv += ori;
+        ++nb_bounces;
+        double rho = var->scatter_rho, mr = var->scatter_mr;
+        double b = rho * sqrt(rng.rnd());
+        double a = sqrt(rho * rho - b * b);
+        double ctg = 1.0 / (2.0 * a * b) * (rho * rho * mr + b * b - a * a);
+        double th = atan(1.0 / ctg);
+        double az = 2.0 * kPi * rng.rnd();
+        d3 rv = dir.x != 0.0 ? mk(0, 0, 1) : (dir.y != 0.0 ? mk(1, 0, 1) : mk(1, 0, 0));
+        d3 un = unitv(cross(dir, rv)), vn = unitv(cross(dir, un));
+        double st = sin(th), ct = cos(th), sa, ca;
+        sincos(az, &sa, &ca);
+        dir = add(add(scl(st * ca, un), scl(st * sa, vn)), scl(ct, dir));
+        double q = mr * rho / b * st;
+        vel = sqrt(vel * vel / (1.0 + q * q));
+        scatter_path = dist + var->scatter_mfp * -log(rng.rnd());
+        trace(-1, 8);
+        return !(var->scatter_cutoff && vel < var->scatter_cut);
+    }
+
+    // Add this lane's global counters to its variant and clear them.
+    __device__ void flush_lane() {
+        if (var == nullptr) return;
+        DGlobal* gl = P.glob + var->glob_index;
+        if (acc_hits) atomic_addu(&gl->nb_mc_hit, acc_hits);
+        if (acc_des) atomic_addu(&gl->nb_desorbed, acc_des);
+        if (acc_hit_equiv != 0.0) atomic_addd(&gl->nb_hit_equiv, acc_hit_equiv);
+        if (acc_abs_equiv != 0.0) atomic_addd(&gl->nb_abs_equiv, acc_abs_equiv);
+        if (acc_leak) atomic_addu(&gl->nb_leak, acc_leak);
+        if (acc_dist_total != 0.0) atomic_addd(&gl->dist_total, acc_dist_total);
+        if (acc_dist_full != 0.0) atomic_addd(&gl->dist_full, acc_dist_full);
+        if (acc_decay) atomic_addu(&gl->nb_decayed, acc_decay);
+        acc_hits = acc_des = acc_leak = acc_decay = 0;
+        acc_hit_equiv = acc_abs_equiv = acc_dist_total = acc_dist_full = 0.0;
+    }
+
+    // One event of an alive particle (one SimulationMCStep iteration).
+    // Returns false when the particle ends.
+    __device__ bool step() {
+        constexpr bool TR = (FEAT & FEAT_TRANSPARENT) != 0;
+        ++nb_events;
+        int32_t root = P.roots[var->root_base + (uint32_t)structure];
+        HitRec h = traverse<TR>(P, root, pos, dir, last_hit, 1e300, stk, stride, P.stack_size, &rng, nb_events, var,
+                                time, false);
+        if (h.facet < 0) {  // leak
+            ++acc_leak;
+            unsigned long long at = atomicAdd(P.leak_count, 1ull);
+            if (at < P.leak_cap) {
+                double* L = P.leaks + at * 6;
+                L[0] = pos.x; L[1] = pos.y; L[2] = pos.z; L[3] = dir.x; L[4] = dir.y; L[5] = dir.z;
+            }
+            trace(-1, 6);
+            return false;
+        }
+        double travel = h.t;
+        int ev = 0;
+        if constexpr ((FEAT & FEAT_VOLUME) != 0) {
+            if (!var->calc_const_flow) { double dl = vel * 100.0 * (var->latest - time); if (dl < travel) { travel = dl; ev = 1; } }
+            if (var->enable_decay) { double dd = vel * 100.0 * (decay_t - time); if (dd < travel) { travel = dd; ev = 2; } }
+            if (var->scatter_enabled) { double ds = scatter_path - dist; if (ds < travel) { travel = ds; ev = 3; } }
+        }
+        if constexpr (TR) {
+            if (var->flags & FEAT_TRANSPARENT) {
+                // second walk: every transparent pass closer than `travel`
+                Ray32 r = make_ray32(pos, dir);
+                float tbf = widen(travel);
+                uint32_t sp = 0;
+                int32_t node = root;
+                double save_u = hu, save_v = hv;
+                for (;;) {
+                    const BvhNode* nd = P.nodes + node;
+                    float tn[4];
+                    uint32_t msk = node_hits(nd, r, tbf, tn);
+                    for (int c = 0; c < 4; ++c) {
+                        if (!(msk & (1u << c))) continue;
+                        int32_t ch = nd->child[c];
+                        if (ch >= 0) {
+                            if (sp < P.stack_size) { stk[(size_t)sp * stride] = ch; ++sp; }
+                            continue;
+                        }
+                        uint32_t first = (uint32_t)(~ch), cnt = nd->count[c];
+                        for (uint32_t kk = 0; kk < cnt; ++kk) {
+                            int32_t f = P.leaf_facets[first + kk];
+                            if (f == last_hit) continue;
+                            const DFacet* F = P.facets + f;
+                            uint32_t fl = F->flags;
+                            if (fl & FF_OPAQUE) continue;
+                            double t, u, v;
+                            if (!facet_test(F, P.poly, pos, dir, t, u, v)) continue;
+                            if (!(t < travel)) continue;
+                            double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
+                            bool hard = op >= 1.0 || (op > 0.0 && rng.coin(nb_events, (uint32_t)local(f)) < op);
+                            if (hard) continue;
+                            hu = u; hv = v;
+                            transparent_pass(f, t);
+                        }
+                    }
+                    if (sp == 0) break;
+                    --sp;
+                    node = stk[(size_t)sp * stride];
+                }
+                hu = save_u; hv = save_v;
+            }
+        }
+        pos = add(pos, scl(travel, dir));
+        time += travel / 100.0 / vel;
+        if (ev == 0) {
+            int32_t f = h.facet;
+            const DPhys& ph = P.phys[f];
+            hu = h.u; hv = h.v;
+            acc_dist_total += travel; acc_dist_full += travel; dist += travel;
+            if constexpr ((FEAT & FEAT_TELEPORT) != 0) {
+                if (ph.teleport_dest != 0) { teleport(f); return true; }
+            }
+            double s = sticking(ph);
+            bool lowflux = false;
+            if constexpr ((FEAT & FEAT_LOWFLUX) != 0) lowflux = var->low_flux != 0;
+            if (!lowflux) {
+                if (s == 1.0 || (s > 0.0 && rng.rnd() < s)) { record_absorb(f); trace(f, 2); return false; }
+                bounce(f);
+                return true;
+            }
+            if (s > 0.0) {
+                double w0 = ori;
+                ori *= s;
+                record_absorb(f);
+                ori = w0 * (1.0 - s);
+                if constexpr ((FEAT & FEAT_ADJOINT) != 0) { if (ph.adj_group >= 0) adj_cnt[ph.adj_group] += 1; }
+            } else {
+                ori *= (1.0 - s);
+            }
+            if (ori > var->low_flux_cutoff) { bounce(f); return true; }
+            trace(f, 2);
+            return false;
+        }
+        if (ev == 1 || ev == 2) {
+            acc_dist_total += travel;
+            ++acc_decay;
+            trace(-1, 7);
+            return false;
+        }
+        acc_dist_total += travel; acc_dist_full += travel; dist += travel;
+        bool ok = scatter();
+        last_hit = -1;
+        if (!ok) { ++acc_decay; return false; }
+        return true;
+    }
+};
+
+__device__ __forceinline__ uint32_t find_variant(const uint64_t* prefix, uint32_t n, uint64_t g) {
+    uint32_t lo = 0, hi = n;  // largest v with prefix[v] <= g
+    while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (prefix[mid] <= g) lo = mid; else hi = mid; }
+    return lo;
+}
+
+template <uint32_t FEAT>
+__global__ void __launch_bounds__(256) k_trace(TraceParams P) {
+    extern __shared__ int32_t lds_stack[];
+    const uint32_t lane = __lane_id();
+    int32_t* stk = lds_stack + threadIdx.x;
+    Tracer<FEAT> T(P, stk, blockDim.x);
+    if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+        T.adj_cnt = P.adj_counts + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * P.adj_groups;
+    }
+    bool alive = false, done = false;
+    uint32_t vi = 0;
+    uint64_t events_left = 0;
+    for (;;) {
+        bool need = !alive && !done;
+        uint64_t mask = __ballot(need);
+        if (mask) {
+            uint32_t cnt = (uint32_t)__popcll(mask);
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) {
+                int stop = *P.stop_flag;
+                base = stop ? P.total : atomicAdd(P.next_particle, (unsigned long long)cnt);
+            }
+            base = __shfl(base, leader, 64);
+            bool started = false;
+            int32_t src = -1;
+            if (need) {
+                uint64_t g = base + mbcnt(mask);
+                if (g < P.total) {
+                    vi = P.n_variants > 1 ? find_variant(P.var_prefix, P.n_variants, g) : 0;
+                    if (T.var != P.variants + vi) T.flush_lane();
+                    T.var = P.variants + vi;
+                    ++T.acc_des;
+                    uint64_t lp = g - P.var_prefix[vi] + T.var->pid_base;
+                    src = T.start(lp);
+                    started = true;
+                    alive = true;
+                    events_left = P.max_events ? P.max_events : ~0ull;
+                } else {
+                    done = true;
+                }
+            }
+            // desorption counters: lanes sharing the leader's source facet
+            // add through one set of atomics
+            uint64_t smask = __ballot(started);
+            if (smask) {
+                const DPhys* ph = started ? &P.phys[src] : nullptr;
+                double ov = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+                int m = -1;
+                if (started) {
+                    ov = T.vel * fabs(dot(T.dir, ld3(P.facets[src].N)));
+                    w1 = 2.0 / ov; w2 = T.var->c_factor * ov; w3 = 1.0 / T.vel;
+                    m = T.moment(T.time);
+                }
+                uint32_t sl = (uint32_t)__ffsll((unsigned long long)smask) - 1;
+                int32_t key = __shfl(src, sl, 64);
+                bool grp = started && src == key;
+                uint64_t gmask = __ballot(grp);
+                double s1 = wave_sum(grp ? w1 : 0.0), s2 = wave_sum(grp ? w2 : 0.0), s3 = wave_sum(grp ? w3 : 0.0);
+                if (grp && lane == sl) {
+                    mfg_facet_counters* c = T.fc(*ph, 0);
+                    atomic_addu(&c->nb_desorbed, (unsigned long long)__popcll(gmask));
+                    atomic_addd(&c->sum_1_per_ort_velocity, s1);
+                    atomic_addd(&c->sum_v_ort, s2);
+                    atomic_addd(&c->sum_1_per_velocity, s3);
+                }
+                if (started && !grp) {
+                    mfg_facet_counters* c = T.fc(*ph, 0);
+                    atomic_addu(&c->nb_desorbed, 1ull);
+                    atomic_addd(&c->sum_1_per_ort_velocity, w1);
+                    atomic_addd(&c->sum_v_ort, w2);
+                    atomic_addd(&c->sum_1_per_velocity, w3);
+                }
+                if (started) T.desorption_records(src, m, ov);
+            }
+        }
+        if (!__ballot(!done)) break;
+        if (alive) {
+            alive = T.step();
+            if (alive && --events_left == 0) alive = false;
+        }
+    }
+    // flush lane accumulators: one set of atomics per wave for a single
+    // variant, per lane otherwise
+    if (P.n_variants == 1) {
+        double he = wave_sum(T.acc_hit_equiv), ae = wave_sum(T.acc_abs_equiv);
+        double d1 = wave_sum(T.acc_dist_total), d2 = wave_sum(T.acc_dist_full);
+        unsigned long long hits = wave_sum_u(T.acc_hits), des = wave_sum_u(T.acc_des);
+        unsigned long long lk = wave_sum_u(T.acc_leak), dc = wave_sum_u(T.acc_decay);
+        if (lane == 0) {
+            DGlobal* gl = P.glob + P.variants[0].glob_index;
+            if (hits) atomic_addu(&gl->nb_mc_hit, hits);
+            if (des) atomic_addu(&gl->nb_desorbed, des);
+            if (he != 0.0) atomic_addd(&gl->nb_hit_equiv, he);
+            if (ae != 0.0) atomic_addd(&gl->nb_abs_equiv, ae);
+            if (lk) atomic_addu(&gl->nb_leak, lk);
+            if (d1 != 0.0) atomic_addd(&gl->dist_total, d1);
+            if (d2 != 0.0) atomic_addd(&gl->dist_full, d2);
+            if (dc) atomic_addu(&gl->nb_decayed, dc);
+        }
+    } else {
+        T.flush_lane();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
+    extern __shared__ int32_t lds_stack[];
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    TraceParams tp{};
+    tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaf_facets = p.leaf_facets;
+    d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
+    int32_t skip = p.last_hit ? p.last_hit[i] : -1;
+    if (skip >= 0) skip += (int32_t)p.facet_base;
+    HitRec h = traverse<false>(tp, p.root, P, D, skip, 1e300, lds_stack + threadIdx.x, blockDim.x, p.stack_size,
+                               nullptr, 0, nullptr, 0.0, true);
+    p.out_facet[i] = h.facet >= 0 ? h.facet - (int32_t)p.facet_base : -1;
+    p.out_t[i] = h.facet >= 0 ? h.t : 0.0;
+    p.out_u[i] = h.u;
+    p.out_v[i] = h.v;
+}
+
+__global__ void k_zero(uint64_t* p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+
+__global__ void k_backward(const double* jac, const double* up, double* grad, uint32_t nt, uint32_t ng, double scale) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    double s = 0.0;
+    for (uint32_t t = 0; t < nt; ++t) s += up[t] * jac[(size_t)t * ng + g];
+    grad[g] = s * scale;
+}
+
+constexpr uint32_t kLean = 0;
+constexpr uint32_t kFull = FEAT_ALL;
+constexpr uint32_t kFullAdj = FEAT_ALL | FEAT_ADJOINT;
+
+}  // namespace
+
+uint32_t mfg_trace_kernel_features(uint32_t model_features) {
+    if (model_features & FEAT_ADJOINT) return kFullAdj;
+    if (model_features & FEAT_ALL) return kFull;
+    return kLean;
+}
+
+int mfg_trace_occupancy(uint32_t features, int block, size_t lds) {
+    int n = 0;
+    hipError_t e;
+    switch (mfg_trace_kernel_features(features)) {
+        case kLean: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kLean>, block, lds); break;
+        case kFull: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kFull>, block, lds); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kFullAdj>, block, lds); break;
+    }
+    return e == hipSuccess ? n : 1;
+}
+
+int mfg_launch_trace(const TraceParams& p, uint32_t features, int grid, int block, void* stream) {
+    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t);
+    hipStream_t s = (hipStream_t)stream;
+    switch (mfg_trace_kernel_features(features)) {
+        case kLean: hipLaunchKernelGGL(k_trace<kLean>, dim3(grid), dim3(block), lds, s, p); break;
+        case kFull: hipLaunchKernelGGL(k_trace<kFull>, dim3(grid), dim3(block), lds, s, p); break;
+        default: hipLaunchKernelGGL(k_trace<kFullAdj>, dim3(grid), dim3(block), lds, s, p); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mfg_launch_intersect(const IsectParams& p, void* stream) {
+    const int block = 256;
+    uint64_t grid = (p.n + block - 1) / block;
+    if (grid == 0) return 0;
+    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t);
+    hipLaunchKernelGGL(k_intersect, dim3((uint32_t)grid), dim3(block), lds, (hipStream_t)stream, p);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mfg_launch_zero(void* ptr, uint64_t bytes, void* stream) {
+    uint64_t n = bytes / 8;
+    if (n == 0) return 0;
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(k_zero, dim3((uint32_t)grid), dim3(256), 0, (hipStream_t)stream, (uint64_t*)ptr, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mfg_launch_backward(const double* jac, const double* up, double* grad, uint32_t nt, uint32_t ng, double scale,
+                        void* stream) {
+    hipLaunchKernelGGL(k_backward, dim3((ng + 63) / 64), dim3(64), 0, (hipStream_t)stream, jac, up, grad, nt, ng, scale);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
