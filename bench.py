@@ -1,0 +1,161 @@
+"""Throughput benchmark of the MI355X test-particle tracer.
+
+Workload (N=1): the synthetic beam pipe -- straight cylinder R 4 cm, L 2 m,
+~100k triangles, 8 side ports (R 2 cm, 20 cm) ending in pump disks of
+sticking 0.05, 6 NEG bands of sticking 0.01, outgassing uniform per wall
+area, 256x256 textures on 16 facets and 90x180 angle maps on 4 facets.
+A step is one launch that desorbs and traces `--des-per-step` particles to
+the end; the full 1e8-desorption job is 100 steps of 1e6.
+
+Prints ONE JSON line: hits/s (reference definition: bounces + absorptions +
+volume scatters per wall second) over K timed steps, with the roofline
+figure of the trace kernel (algorithmic bytes per hit from the reference-
+equivalent BVH, HIP-event kernel time) and the CPU oracle timed on this
+host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per hit (one wall event), from the reference-equivalent
+# BVH (binary SAH, <= 2 facets per leaf): 32 B per visited node (FP32 box +
+# child word), 48 B per tested facet (plane, origin, edge vectors as FP32
+# equivalents), 96 B of the hit facet's parameters, 16 B per tally field
+# updated (read + write of 8 B) with 5 fields per bounce.
+NODE_BYTES, PRIM_BYTES, FACET_BYTES, FIELD_BYTES, FIELDS_PER_HIT = 32, 48, 96, 16, 5
+
+
+def algorithmic_bytes_per_hit(model, n_rays=20000, seed=5):
+    """Mean BVH nodes and facet tests per ray on the oracle's BVH, from
+    particles' own rays (desorption + one diffuse bounce positions)."""
+    import numpy as np
+
+    from oracle.mfcpu import CpuModel
+    cm = CpuModel(model)
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(1, 199, n_rays)
+    r = rng.uniform(0, 3.9, n_rays)
+    a = rng.uniform(0, 2 * math.pi, n_rays)
+    o = np.stack([r * np.cos(a), r * np.sin(a), z], 1)
+    d = rng.normal(size=(n_rays, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nodes, prims = cm.bvh_stats(o, d)
+    b = NODE_BYTES * nodes + PRIM_BYTES * prims + FACET_BYTES + FIELD_BYTES * FIELDS_PER_HIT
+    return b, nodes, prims
+
+
+def cpu_baseline(model, target_s=15.0):
+    """The CPU oracle (one tracer per OpenMP thread, thread-local tallies,
+    the reference's CPU parallelism) on a bounded sample of the workload."""
+    from oracle.mfcpu import CpuModel
+    cm = CpuModel(model)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", threads))
+    n = 2000
+    t0 = time.perf_counter()
+    t = cm.run(n, seed=1, threads=threads)
+    dt = time.perf_counter() - t0
+    hits = int(t.glob[0]["nb_mc_hit"])
+    n2 = int(max(2000, min(2_000_000, n * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    t = cm.run(n2, seed=2, first=n, threads=threads)
+    dt = time.perf_counter() - t0
+    hits = int(t.glob[0]["nb_mc_hit"])
+    return {"value": hits / dt, "unit": "hits/s", "cores": threads, "kind": "port",
+            "sample": f"{n2} desorptions of the same beam-pipe model ({hits} hits, {dt:.1f} s), oracle/mfcpu.cpp"}
+
+
+def traffic_from_profile():
+    """HBM bytes per trace launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration)."""
+    p = os.path.join(ROOT, "profiles", "trace_beam_pipe_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--des-per-step", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build runs on a single GPU (see DESIGN.md for sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from molflow_sources_amd import geometry
+    from molflow_sources_amd.simulation import GpuSimulation
+
+    model = geometry.beam_pipe_model()
+    sim = GpuSimulation(0)
+    sim.load(model)
+    sim.rebuild_accel()
+    sim.set_launch(args.block, args.blocks_per_cu)
+    sim.reset(seed=42)
+    max_events = 50_000_000
+    for _ in range(args.warmup):
+        sim.run(args.des_per_step, max_events=max_events)
+    sim.reset(seed=42)
+    hits = 0
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = sim.run(args.des_per_step, max_events=max_events)   # synchronous: returns after the device is done
+        hits += r.hits
+        kernel_ms += r.kernel_ms
+    wall = time.perf_counter() - t0
+    t = sim.read()
+    assert t.glob[0]["nb_desorbed"] == args.steps * args.des_per_step
+    value = hits / wall
+    b_hit, nodes, prims = algorithmic_bytes_per_hit(model)
+    launch_s = kernel_ms / 1e3 / args.steps
+    hits_per_launch = hits / args.steps
+    achieved = b_hit * hits_per_launch / launch_s / 1e9
+    out = {
+        "metric": "MC hits/s on 1 MI355X (plain tracer, config-2 beam pipe)",
+        "value": value,
+        "unit": "hits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "beam_pipe_100k_triangles_8_ports_16_textures_4_anglemaps",
+                   "facets": model.n_facets, "desorptions_per_step": args.des_per_step,
+                   "hits_per_desorption": hits / (args.steps * args.des_per_step),
+                   "seed": 42, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "bytes_per_hit": b_hit, "bvh_nodes_per_ray": nodes, "facet_tests_per_ray": prims,
+                     "kernel_ms_per_launch": launch_s * 1e3},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(model),
+    }
+    if out["cpu_baseline"]:
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

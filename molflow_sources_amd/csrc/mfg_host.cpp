@@ -1008,25 +1008,28 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     TraceParams p = make_params(c, total, max_events);
     std::vector<uint64_t> key(n_des, n_des + nv);
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
-    auto enqueue = [&]() {
+    // the events bracket the trace kernel alone (eager path) or the whole
+    // graph (replay path)
+    auto enqueue = [&](bool events) {
         HIPCHK(hipMemcpyAsync(c->variants.p, c->pin_vars, nv * sizeof(DVariant), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(c->prefix.p, c->pin_prefix, (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
         if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s)) throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
+        if (events) HIPCHK(hipEventRecord(c->ev0, s));
         if (mfg_launch_trace(p, feats, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
     };
-    HIPCHK(hipEventRecord(c->ev0, s));
     if (c->graph_on) {
+        HIPCHK(hipEventRecord(c->ev0, s));
         if (!c->graph_exec || c->graph_key != key) {
             reset_graph(c);
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try { enqueue(); } catch (...) { hipGraph_t g; (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); throw; }
+            try { enqueue(false); } catch (...) { hipGraph_t g = nullptr; (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); throw; }
             HIPCHK(hipStreamEndCapture(s, &c->graph));
             HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
             c->graph_key = key;
         }
         HIPCHK(hipGraphLaunch(c->graph_exec, s));
     } else {
-        enqueue();
+        enqueue(true);
     }
     HIPCHK(hipEventRecord(c->ev1, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -1392,7 +1392,7 @@ double mfcpu_latest_moment(const mfcpu_model* m) { return m->M.latest; }
 
 int mfcpu_adjoint_run(mfcpu_model* mm, uint64_t seed, uint64_t first, uint64_t n_des, const int32_t* group,
                       uint32_t G, const double* gs, const int32_t* targets, uint32_t nT, int threads,
-                      double* j0, double* jac) {
+                      double* j0, double* jac, uint64_t max_events) {
     if (!mm || !mm->M.s.low_flux_mode) return MFG_ERR_INVALID;
     const Model& M = mm->M;
     int nt = 1;
@@ -1420,7 +1420,7 @@ int mfcpu_adjoint_run(mfcpu_model* mm, uint64_t seed, uint64_t first, uint64_t n
         tr.ncoll.assign(G, 0);
         std::vector<TPass> passes;
 #pragma omp for schedule(dynamic, 256)
-        for (int64_t i = 0; i < (int64_t)n_des; ++i) tr.run_particle(seed, first + (uint64_t)i, 0, passes);
+        for (int64_t i = 0; i < (int64_t)n_des; ++i) tr.run_particle(seed, first + (uint64_t)i, max_events, passes);
     }
     for (uint32_t t = 0; t < nT; ++t) { j0[t] = 0; for (uint32_t g = 0; g < G; ++g) jac[(size_t)t * G + g] = 0; }
     for (auto& a : adjs) {

@@ -70,7 +70,7 @@ double mfcpu_latest_moment(const mfcpu_model* m);
 int mfcpu_adjoint_run(mfcpu_model* m, uint64_t seed, uint64_t first_particle, uint64_t n_des,
                       const int32_t* group_of_facet, uint32_t n_groups, const double* group_sticking,
                       const int32_t* targets, uint32_t n_targets, int threads,
-                      double* j0, double* jac);
+                      double* j0, double* jac, uint64_t max_events);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,7 @@ def lib() -> C.CDLL:
             getattr(L, n).restype = C.c_double
             getattr(L, n).argtypes = [vp]
         L.mfcpu_adjoint_run.argtypes = [vp, u64, u64, u64, C.POINTER(i32), C.c_uint32, dp, C.POINTER(i32),
-                                        C.c_uint32, C.c_int, dp, dp]
+                                        C.c_uint32, C.c_int, dp, dp, u64]
         _lib = L
     return _lib
 
@@ -165,14 +165,14 @@ class CpuModel:
     def latest_moment(self) -> float:
         return lib().mfcpu_latest_moment(self.h)
 
-    def adjoint(self, n_des, group, sticking, targets, seed=42, first=0, threads=0):
+    def adjoint(self, n_des, group, sticking, targets, seed=42, first=0, threads=0, max_events=0):
         group = np.ascontiguousarray(group, dtype=np.int32)
         sticking = np.ascontiguousarray(sticking, dtype=np.float64)
         targets = np.ascontiguousarray(targets, dtype=np.int32)
         j0 = np.zeros(len(targets))
         jac = np.zeros((len(targets), len(sticking)))
         rc = lib().mfcpu_adjoint_run(self.h, seed, first, n_des, _ip(group), len(sticking), _dp(sticking),
-                                     _ip(targets), len(targets), threads, _dp(j0), _dp(jac))
+                                     _ip(targets), len(targets), threads, _dp(j0), _dp(jac), max_events)
         if rc:
             raise RuntimeError("mfcpu_adjoint_run failed (low-flux mode required)")
         return j0, jac

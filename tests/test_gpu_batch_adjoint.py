@@ -1,0 +1,221 @@
+"""Batched variants, launch-shape independence, hipGraph replays, stop
+requests and the sticking adjoint. Needs an MI355X."""
+import math
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from molflow_sources_amd import geometry
+from oracle.mfcpu import CpuModel
+
+pytestmark = pytest.mark.gpu
+
+MAXEV = 5_000_000
+
+
+@pytest.fixture(scope="module")
+def sim():
+    from molflow_sources_amd.simulation import GpuSimulation
+    s = GpuSimulation(0)
+    yield s
+    s.close()
+
+
+def int_tallies(t):
+    return (int(t.glob[0]["nb_mc_hit"]), int(t.glob[0]["nb_desorbed"]), t.facets["nb_mc_hit"].astype(np.int64).copy(),
+            t.facets["nb_desorbed"].astype(np.int64).copy())
+
+
+def test_batched_variants_equal_plain_runs(sim):
+    vs = geometry.variant_models(6, min_facets=2000, max_facets=30000, seed=4)
+    nd = [3000, 500, 2000, 800, 1500, 1000]
+    seeds = [101, 102, 103, 104, 105, 106]
+    sim.load_variants(vs)
+    sim.rebuild_accel()
+    sim.reset()
+    r = sim.run_variants(nd, seeds, max_events=MAXEV)
+    assert r.desorbed == sum(nd)
+    from molflow_sources_amd.simulation import GpuSimulation
+    p = GpuSimulation(0)
+    for v, m in enumerate(vs):
+        tb = sim.read(variant=v)
+        p.load(m)
+        p.rebuild_accel()
+        p.reset(seed=seeds[v])
+        p.run(nd[v], max_events=MAXEV)
+        tp = p.read()
+        a, b = int_tallies(tb), int_tallies(tp)
+        # identical RNG streams: the batched launch reproduces the plain run
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    p.close()
+
+
+def test_launch_shape_does_not_change_results(sim):
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=4, n_anglemaps=2)
+    sim.load(m)
+    sim.rebuild_accel()
+    res = []
+    for block, per_cu in ((256, 0), (64, 3), (128, 1)):
+        sim.set_launch(block, per_cu)
+        sim.reset(seed=9)
+        sim.run(4000, max_events=MAXEV)
+        t = sim.read()
+        res.append(int_tallies(t) + (t.angle_maps.copy(), t.textures["count_equiv"].copy()))
+    sim.set_launch(256, 0)
+    for other in res[1:]:
+        assert res[0][0] == other[0] and np.array_equal(res[0][2], other[2]) and np.array_equal(res[0][4], other[4])
+        assert np.allclose(res[0][5], other[5], rtol=1e-9, atol=1e-9)
+
+
+def test_runs_continue_particle_streams(sim):
+    m = geometry.tube_model()
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=3)
+    sim.run(5000, max_events=MAXEV)
+    sim.run(5000, max_events=MAXEV)
+    two = int_tallies(sim.read())
+    sim.reset(seed=3)
+    sim.run(10000, max_events=MAXEV)
+    one = int_tallies(sim.read())
+    assert two[0] == one[0] and np.array_equal(two[2], one[2])
+    # resume offset: particles 5000.. of a fresh context equal the second half
+    sim.reset(seed=3, particle_offset=5000)
+    sim.run(5000, max_events=MAXEV)
+    half = int_tallies(sim.read())
+    sim.reset(seed=3)
+    sim.run(5000, max_events=MAXEV)
+    first = int_tallies(sim.read())
+    assert half[0] + first[0] == one[0]
+
+
+def test_graph_replays_match_eager(sim):
+    vs = geometry.variant_models(4, min_facets=2000, max_facets=8000, seed=6)
+    nd, seeds = [1000, 2000, 500, 1500], [1, 2, 3, 4]
+    sim.load_variants(vs)
+    sim.rebuild_accel()
+    sim.reset()
+    for _ in range(3):
+        sim.run_variants(nd, seeds, max_events=MAXEV)
+    eager = [int_tallies(sim.read(variant=v)) for v in range(4)]
+    sim.enable_graph(True)
+    sim.reset()
+    for _ in range(3):
+        sim.run_variants(nd, seeds, max_events=MAXEV)
+        time.sleep(0.01)
+    graph = [int_tallies(sim.read(variant=v)) for v in range(4)]
+    sim.enable_graph(False)
+    for a, b in zip(eager, graph):
+        assert a[0] == b[0] and np.array_equal(a[2], b[2])
+
+
+def test_request_stop_ends_run_early(sim):
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=1)
+    t = threading.Timer(0.3, sim.request_stop)
+    t.start()
+    r = sim.run(50_000_000, max_events=MAXEV)
+    t.join()
+    assert r.stopped == 1 and 0 < r.desorbed < 50_000_000
+    g = sim.read().glob[0]
+    # accounting: every started particle ended (absorbed, leaked or dropped)
+    assert g["nb_desorbed"] == r.desorbed
+    assert abs(g["nb_abs_equiv"] - r.desorbed) <= 0
+
+
+def adjoint_model():
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+    m.settings.low_flux_mode = 1
+    m.settings.low_flux_cutoff = 1e-6
+    g, s, t = geometry.adjoint_setup(m, n_groups=8, n_targets=16, seed=5)
+    return m, g, s, t
+
+
+def test_adjoint_matches_oracle_path_scores(sim):
+    """Same seed, first 40 events of every particle: GPU and oracle follow
+    the same paths, so J0 and the path-score Jacobian agree closely."""
+    m, grp, base, tg = adjoint_model()
+    n = 4000
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=13)
+    sim.adjoint_setup(grp, tg)
+    sim.adjoint_set_sticking(base)
+    sim.run(n, max_events=40)
+    j0, jac = sim.adjoint_read()
+    cm = CpuModel(m)
+    cj0, cjac = cm.adjoint(n, grp, base, tg, seed=13, max_events=40)
+    assert np.abs(j0 - cj0).sum() <= 0.01 * np.abs(cj0).sum()
+    assert np.abs(jac - cjac).sum() <= 0.01 * np.abs(cjac).sum()
+    up = np.random.default_rng(1).normal(size=len(tg))
+    g = sim.adjoint_backward(up)
+    assert np.allclose(g, up @ jac / n, rtol=1e-9, atol=1e-12)
+
+
+def test_adjoint_full_paths_statistically(sim):
+    """Full low-flux paths: GPU and oracle agree within 4 sigma (batch means)."""
+    m, grp, base, tg = adjoint_model()
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.adjoint_setup(grp, tg)
+    sim.adjoint_set_sticking(base)
+    up = np.random.default_rng(3).normal(size=len(tg))
+
+    def gpu_batches(k, n):
+        out = []
+        for b in range(k):
+            sim.reset(seed=100 + b)
+            sim.adjoint_set_sticking(base)
+            sim.run(n, max_events=MAXEV)
+            j0, jac = sim.adjoint_read()
+            out.append(np.concatenate([j0 / n, up @ jac / n]))
+        return np.array(out)
+
+    cm = CpuModel(m)
+
+    def cpu_batches(k, n):
+        out = []
+        for b in range(k):
+            j0, jac = cm.adjoint(n, grp, base, tg, seed=500 + b)
+            out.append(np.concatenate([j0 / n, up @ jac / n]))
+        return np.array(out)
+
+    G = gpu_batches(8, 20000)
+    Cc = cpu_batches(8, 1000)
+    mg, sg = G.mean(0), G.std(0, ddof=1) / math.sqrt(8)
+    mc, sc = Cc.mean(0), Cc.std(0, ddof=1) / math.sqrt(8)
+    z = (mg - mc) / np.sqrt(sg ** 2 + sc ** 2 + 1e-300)
+    assert np.all(np.abs(z) < 4.5), z
+
+
+def test_adjoint_matches_central_differences(sim):
+    """d J0 / d s_g from the path-score Jacobian vs central finite
+    differences (ds = 1e-3 relative) with common random numbers."""
+    m, grp, base, tg = adjoint_model()
+    n = 20_000
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.adjoint_setup(grp, tg)
+
+    def forward(s):
+        sim.reset(seed=17)
+        sim.adjoint_set_sticking(s)
+        sim.run(n, max_events=MAXEV)
+        return sim.adjoint_read()
+
+    j0, jac = forward(base)
+    up = np.random.default_rng(2).normal(size=len(tg))
+    grad = up @ jac
+    for gi in range(len(base)):
+        ds = 1e-3 * base[gi]
+        sp, sm = base.copy(), base.copy()
+        sp[gi] += ds
+        sm[gi] -= ds
+        fd = (up @ forward(sp)[0] - up @ forward(sm)[0]) / (2 * ds)
+        # common random numbers: the paths are the same, only weights change
+        assert abs(fd - grad[gi]) <= 2e-3 * max(abs(grad[gi]), 1e-3 * np.abs(grad).max()) + 1e-9, (gi, fd, grad[gi])

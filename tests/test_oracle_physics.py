@@ -1,0 +1,108 @@
+"""Closed-form checks of the oracle's physics (emission laws, speeds,
+per-event tallies), each chosen so that a plausible mistake fails it."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from molflow_sources_amd import _abi as A
+from molflow_sources_amd.model import Model, default_facets, default_settings
+from oracle.mfcpu import CpuModel
+
+
+def box_model(desorb_type=A.DES_COSINE, exponent=0.0, src_2sided=0):
+    """Unit cube with inward normals; facet 0 (z=0) desorbs, all facets absorb."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    polys = [[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [1, 5, 6, 2], [2, 6, 7, 3], [3, 7, 4, 0]]
+    F = default_facets(6)
+    F["sticking"] = 1.0
+    F["desorb_type"][0] = desorb_type
+    F["desorb_exponent"][0] = exponent
+    F["outgassing"][0] = 1.0
+    F["is_2sided"][0] = src_2sided
+    s = default_settings()
+    m = Model(V, polys, F, settings=s)
+    return m
+
+
+def emitted_angles(m, n, seed=3):
+    m.settings.log_enabled = 1
+    m.settings.log_facet_id = 0
+    m.settings.log_limit = n
+    cm = CpuModel(m)
+    t = cm.run(n, seed=seed, threads=1)
+    k = int(t.log_count[0])
+    assert k == n
+    return t.log["hit_theta"][:k], t.log["hit_phi"][:k], t
+
+
+@pytest.mark.parametrize("dtype,exponent,cdf", [
+    (A.DES_COSINE, 0.0, lambda c: c ** 2),        # P(cos t <= c) = c^2 (Lambert)
+    (A.DES_UNIFORM, 0.0, lambda c: c),             # uniform over the hemisphere
+    (A.DES_COSINE_N, 3.0, lambda c: c ** 4),       # cos^N: c^(N+1)
+])
+def test_emission_law(oracle_lib, dtype, exponent, cdf):
+    th, ph, _ = emitted_angles(box_model(dtype, exponent), 40000)
+    c = np.cos(th)
+    assert np.all(c > 0)   # into the volume (facet normal side)
+    assert stats.kstest(c, cdf).pvalue > 1e-3
+    # azimuth uniform on (-pi, pi]
+    assert stats.kstest((ph + math.pi) / (2 * math.pi), "uniform").pvalue > 1e-3
+
+
+def test_two_sided_source_emits_both_sides(oracle_lib):
+    th, _, _ = emitted_angles(box_model(src_2sided=1), 20000)
+    frac_back = np.mean(np.cos(th) < 0)
+    assert abs(frac_back - 0.5) < 4 * math.sqrt(0.25 / 20000)
+
+
+def test_cosine_mean(oracle_lib):
+    th, _, _ = emitted_angles(box_model(), 40000)
+    assert abs(np.mean(np.cos(th)) - 2 / 3) < 4 * math.sqrt(1 / 18 / 40000)
+
+
+def test_maxwell_speed_mean(oracle_lib):
+    # wall-collision (flux-weighted) Maxwell speeds at T: mean = sqrt(9 pi k T / (8 m))
+    m = box_model()
+    m.facets["temperature"] = 293.15
+    _, _, t = emitted_angles(m, 40000)
+    v = t.log["velocity"][:40000]
+    mass = 28 * 1.67e-27
+    a2 = 1.38e-23 * 293.15 / mass
+    mean = 0.75 * math.sqrt(2 * math.pi * a2)
+    assert abs(v.mean() / mean - 1) < 0.01
+
+
+def test_event_tallies_of_a_single_absorption(oracle_lib):
+    # every particle is absorbed at its first hit: per facet nbDesorbed,
+    # nbAbsEquiv, nbMCHit and the velocity sums follow IncreaseFacetCounter
+    m = box_model()
+    m.settings.use_maxwell = 0
+    n = 20000
+    t = CpuModel(m).run(n, seed=1)
+    f = t.facets[0]
+    g = t.glob[0]
+    assert g["nb_desorbed"] == n and g["nb_mc_hit"] == n and g["nb_abs_equiv"] == n
+    assert f["nb_desorbed"][0] == n and f["nb_mc_hit"][0] == 0
+    assert f["nb_abs_equiv"][1:].sum() == n and f["nb_mc_hit"][1:].sum() == n
+    v = 145.469 * math.sqrt(293.15 / 28.0)   # non-Maxwell speed at desorption
+    # sum_1_per_velocity: (hitEquiv + desorb) / v -> n/v on the source, n/v on absorbers
+    assert abs(f["sum_1_per_velocity"][0] - n / v) < 1e-9 * n / v
+    assert abs(f["sum_1_per_velocity"][1:].sum() - n / v) < 1e-9 * n / v
+    # sum_v_ort on the source: 1.1781 * v * <cos t> with <cos t> = 2/3
+    assert abs(f["sum_v_ort"][0] / n / (1.1781 * v * 2 / 3) - 1) < 0.01
+    # sum_1_per_ort_velocity on the source: 2/(v cos t) summed (heavy tailed)
+    assert f["sum_1_per_ort_velocity"][0] > 2 * n / v
+
+
+def test_low_flux_conserves_desorbed_weight(oracle_lib):
+    # with low-flux weights, absorbed weight + weight killed below the cutoff = desorbed
+    from molflow_sources_amd import geometry
+    m = geometry.tube_model(wall_sticking=0.02)
+    m.settings.low_flux_mode = 1
+    m.settings.low_flux_cutoff = 1e-9
+    n = 20000
+    t = CpuModel(m).run(n, seed=2)
+    g = t.glob[0]
+    assert abs(g["nb_abs_equiv"] / n - 1) < 1e-6
