@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs on a single GPU (see DESIGN.md for sharding)",
@@ -103,6 +105,16 @@ def main():
 
     from molflow_sources_amd import geometry
     from molflow_sources_amd.simulation import GpuSimulation
+
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     model = geometry.beam_pipe_model()
     sim = GpuSimulation(0)

@@ -362,6 +362,23 @@ int mfg_read_variant_tallies(mfg_context* ctx, uint32_t variant, const mfg_tally
 /* Launch shape used by runs: threads per block and blocks per CU
  * (0 = defaults / autotuned). */
 int mfg_set_launch(mfg_context* ctx, uint32_t block_threads, uint32_t blocks_per_cu);
+/* Occupancy bound of the trace kernel: waves per SIMD (1..4; 0 = default).
+ * Higher bounds cap registers (cold state spills) to hide more latency. */
+int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);
+
+/* Launch autotuner: times short probe runs of n_probe desorptions for every
+ * compiled occupancy bound and block size on the loaded model(s), keeps the
+ * fastest, then zeroes the tallies and puts the particle streams back where
+ * they were, so the probes leave no trace in later results. */
+typedef struct mfg_tune_result {
+    uint32_t waves;          /* chosen waves per SIMD */
+    uint32_t block;          /* threads per block */
+    uint32_t blocks_per_cu;  /* resident blocks per CU at that shape */
+    uint32_t reserved;
+    double hits_per_s;       /* probe throughput of the choice */
+} mfg_tune_result;
+int mfg_autotune(mfg_context* ctx, uint64_t n_probe, mfg_tune_result* out);
+
 /* Capture the variant run in a hipGraph and replay it on later calls. */
 int mfg_enable_graph(mfg_context* ctx, int enable);
 

@@ -97,6 +97,10 @@ class TallyView(C.Structure):
                 ("leak_count", C.c_void_p)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("waves", _u32), ("block", _u32), ("blocks_per_cu", _u32), ("reserved", _u32), ("hits_per_s", _d)]
+
+
 class StepResult(C.Structure):
     _fields_ = [("status", _i32), ("stopped", _i32), ("desorbed", _u64), ("hits", _u64), ("kernel_ms", _d)]
 
@@ -150,6 +154,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mfg_read_variant_tallies": (C.c_int, [vp, u32, C.POINTER(TallyView)]),
         "mfg_set_launch": (C.c_int, [vp, u32, u32]),
         "mfg_enable_graph": (C.c_int, [vp, C.c_int]),
+        "mfg_set_kernel_variant": (C.c_int, [vp, u32]),
+        "mfg_autotune": (C.c_int, [vp, u64, C.POINTER(TuneResult)]),
         "mfg_adjoint_setup": (C.c_int, [vp, C.POINTER(i32), u32, C.POINTER(i32), u32]),
         "mfg_adjoint_set_sticking": (C.c_int, [vp, dp, u32]),
         "mfg_adjoint_read": (C.c_int, [vp, dp, dp]),
@@ -168,5 +174,6 @@ EXPORTED_SYMBOLS = (
     "mfg_hits_size", "mfg_intersect_batch", "mfg_maxwell_cdf", "mfg_parse_user_moment",
     "mfg_integrated_desorption", "mfg_set_trace", "mfg_get_trace", "mfg_load_variants", "mfg_run_variants",
     "mfg_get_variant_layout", "mfg_read_variant_tallies", "mfg_set_launch", "mfg_enable_graph",
+    "mfg_set_kernel_variant", "mfg_autotune",
     "mfg_adjoint_setup", "mfg_adjoint_set_sticking", "mfg_adjoint_read", "mfg_adjoint_backward",
 )

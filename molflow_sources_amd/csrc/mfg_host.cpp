@@ -655,6 +655,7 @@ struct mfg_context {
     std::string err;
     int n_cu = 0;
     uint32_t block = 256, blocks_per_cu = 0;
+    int waves = 0;                       // occupancy bound of the trace kernel, 0 = default
     std::vector<std::unique_ptr<HostModel>> models;
     std::vector<VariantLayout> layouts;
     std::vector<DVariant> dvars;
@@ -669,6 +670,7 @@ struct mfg_context {
     DevBuf<DPhys> phys;
     DevBuf<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_start, mom_end, tex_inc;
     DevBuf<int32_t> src_facet, leaf_facets, roots;
+    DevBuf<uint32_t> guides;
     DevBuf<BvhNode> nodes;
     DevBuf<DVariant> variants;
     DevBuf<uint64_t> prefix;
@@ -742,6 +744,7 @@ void upload_models(mfg_context* c) {
     std::vector<DPhys> phys;
     std::vector<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_s, mom_e, tex_inc;
     std::vector<int32_t> src_facet;
+    std::vector<uint32_t> guides;
     std::vector<DTable> params, ids;
     c->dvars.clear();
     c->layouts.clear();
@@ -786,6 +789,16 @@ void upload_models(mfg_context* c) {
         log_cap = std::max(log_cap, dv.log_limit);
         dv.maxwell_base = (uint32_t)tab_x.size();
         for (auto& p : M.maxwell) { tab_x.push_back(p.x); tab_y.push_back(p.y); }
+        dv.maxwell_guide = (uint32_t)guides.size();
+        {
+            const double ylast = M.maxwell.back().y;
+            for (uint32_t k = 0; k < MFG_MAXWELL_GUIDE; ++k) {
+                double start = ylast * (double)k / (double)MFG_MAXWELL_GUIDE;
+                uint32_t i = 0;
+                while (i + 2 < M.maxwell.size() && M.maxwell[i + 1].y <= start) ++i;
+                guides.push_back(i);
+            }
+        }
         dv.param_base = (uint32_t)params.size(); dv.n_params = (uint32_t)M.params.size();
         for (auto& p : M.params) {
             params.push_back(DTable{(uint32_t)tab_x.size(), (uint32_t)p.pts.size(), p.logx, p.logy});
@@ -829,7 +842,7 @@ void upload_models(mfg_context* c) {
     hipStream_t s = c->stream;
     c->facets.upload(facets, s); c->phys.upload(phys, s); c->poly.upload(poly, s);
     c->src_cum.upload(src_cum, s); c->src_facet.upload(src_facet, s); c->og_cdf.upload(og_cdf, s);
-    c->tab_x.upload(tab_x, s); c->tab_y.upload(tab_y, s); c->params.upload(params, s); c->ids.upload(ids, s);
+    c->tab_x.upload(tab_x, s); c->tab_y.upload(tab_y, s); c->guides.upload(guides, s); c->params.upload(params, s); c->ids.upload(ids, s);
     c->mom_start.upload(mom_s, s); c->mom_end.upload(mom_e, s); c->tex_inc.upload(tex_inc, s);
     c->fc.alloc(std::max<uint64_t>(fc, 1)); c->force.alloc(std::max<uint64_t>(force, 1));
     c->tex.alloc(std::max<uint64_t>(tex, 1)); c->dir.alloc(std::max<uint64_t>(dir, 1));
@@ -950,7 +963,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.leaf_facets = c->leaf_facets.p; p.roots = c->roots.p; p.variants = c->variants.p;
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size(); p.stack_size = c->stack_size;
     p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p;
-    p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
+    p.guides = c->guides.p; p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
     p.mom_start = c->mom_start.p; p.mom_end = c->mom_end.p;
     p.fc = c->fc.p; p.force = c->force.p; p.tex = c->tex.p; p.tex_inc = c->tex_inc.p; p.dir = c->dir.p;
     p.prof = c->prof.p; p.am = c->am.p; p.gh = c->gh.p; p.fh = c->fh.p; p.glob = c->glob.p;
@@ -966,13 +979,20 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     return p;
 }
 
-void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block) {
+int default_waves(uint32_t feats) {
+    uint32_t k = mfg_trace_kernel_features(feats);
+    if (k & FEAT_ADJOINT) return 1;
+    return (k & FEAT_ALL) ? 2 : 3;
+}
+
+void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& waves) {
     block = (int)c->block;
+    waves = c->waves ? c->waves : default_waves(feats);
     uint32_t per_cu = c->blocks_per_cu;
     if (per_cu == 0) {
         // fill every CU to its occupancy limit (persistent launch)
         size_t lds = (size_t)block * c->stack_size * sizeof(int32_t);
-        int occ = mfg_trace_occupancy(feats, block, lds);
+        int occ = mfg_trace_occupancy(feats, waves, block, lds);
         per_cu = (uint32_t)std::max(1, occ);
     }
     grid = (int)(per_cu * (uint32_t)c->n_cu);
@@ -994,8 +1014,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     uint32_t feats = c->features;
     if (c->trace_n) feats |= FEAT_TRACE;
     if (c->adj_groups) feats |= FEAT_ADJOINT;
-    int grid, block;
-    launch_shape(c, feats, grid, block);
+    int grid, block, waves;
+    launch_shape(c, feats, grid, block, waves);
     if (c->adj_groups) {
         size_t need = (size_t)grid * block * c->adj_groups;
         if (c->adj_counts.n < need) c->adj_counts.alloc(need);
@@ -1008,6 +1028,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     TraceParams p = make_params(c, total, max_events);
     std::vector<uint64_t> key(n_des, n_des + nv);
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
+    key.push_back((uint64_t)waves);
     // the events bracket the trace kernel alone (eager path) or the whole
     // graph (replay path)
     auto enqueue = [&](bool events) {
@@ -1015,7 +1036,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
         HIPCHK(hipMemcpyAsync(c->prefix.p, c->pin_prefix, (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
         if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s)) throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
         if (events) HIPCHK(hipEventRecord(c->ev0, s));
-        if (mfg_launch_trace(p, feats, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
+        if (mfg_launch_trace(p, feats, waves, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
     };
     if (c->graph_on) {
         HIPCHK(hipEventRecord(c->ev0, s));
@@ -1401,6 +1422,51 @@ int mfg_set_launch(mfg_context* c, uint32_t block, uint32_t per_cu) {
         c->block = block ? block : 256;
         c->blocks_per_cu = per_cu;
         reset_graph(c);
+    });
+}
+
+int mfg_set_kernel_variant(mfg_context* c, uint32_t waves) {
+    return guarded(c, [&] {
+        if (waves > 4) throw Err(MFG_ERR_INVALID, "waves per SIMD must be 0..4");
+        c->waves = (int)waves;
+        reset_graph(c);
+    });
+}
+
+int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
+    return guarded(c, [&] {
+        if (!c->accel) throw Err(MFG_ERR_STATE, "acceleration structure not built");
+        if (n_probe == 0) throw Err(MFG_ERR_INVALID, "probe size must be positive");
+        uint32_t feats = c->features | (c->trace_n ? FEAT_TRACE : 0u) | (c->adj_groups ? FEAT_ADJOINT : 0u);
+        int wl[4];
+        int nw = mfg_trace_variants(feats, wl, 4);
+        const uint32_t blocks[] = {128, 256};
+        size_t nv = c->dvars.size();
+        std::vector<uint64_t> nd(nv, std::max<uint64_t>(1, n_probe / nv)), seeds(nv, c->seed);
+        std::vector<uint64_t> save_pid = c->pid_base;
+        double best = -1;
+        mfg_tune_result r{};
+        // one untimed probe loads the code objects
+        c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
+        run_variants(c, nd.data(), seeds.data(), 0, nullptr);
+        for (int i = 0; i < nw; ++i)
+            for (uint32_t b : blocks) {
+                c->waves = wl[i]; c->block = b; c->blocks_per_cu = 0;
+                mfg_step_result sr{};
+                run_variants(c, nd.data(), seeds.data(), 0, &sr);
+                double hs = sr.kernel_ms > 0 ? (double)sr.hits / (sr.kernel_ms * 1e-3) : 0.0;
+                if (hs > best) {
+                    best = hs;
+                    int grid, block, waves;
+                    launch_shape(c, feats, grid, block, waves);
+                    r.waves = (uint32_t)waves; r.block = b; r.blocks_per_cu = (uint32_t)(grid / c->n_cu); r.hits_per_s = hs;
+                }
+            }
+        c->waves = (int)r.waves; c->block = r.block; c->blocks_per_cu = 0;
+        c->pid_base = save_pid;
+        zero_tallies(c);
+        reset_graph(c);
+        if (out) *out = r;
     });
 }
 

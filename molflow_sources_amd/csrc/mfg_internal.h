@@ -18,6 +18,7 @@
 #define MFG_BVH_WIDTH 4
 #define MFG_MAX_STACK 64
 #define MFG_MAX_GROUPS 64
+#define MFG_MAXWELL_GUIDE 128
 
 // feature bits of a trace-kernel instantiation / of a loaded model
 enum : uint32_t {
@@ -147,6 +148,7 @@ struct DVariant {
     uint32_t log_limit;
     // tables (global offsets)
     uint32_t maxwell_base;             // 100 (v,p) pairs
+    uint32_t maxwell_guide;            // MFG_MAXWELL_GUIDE entries into `guides`
     uint32_t param_base, n_params;     // into param table headers
     uint32_t id_base, n_ids;           // into ID table headers
     uint32_t mom_base, n_moments;      // into interval arrays
@@ -185,6 +187,7 @@ struct TraceParams {
     const double* src_cum;            // cumulative source weights
     const int32_t* src_facet;
     const double* og_cdf;
+    const uint32_t* guides;           // guide tables of the Maxwell CDFs
     const double* tab_x;              // table pool (maxwell CDFs, params, IDs)
     const double* tab_y;
     const DTable* params;
@@ -243,10 +246,11 @@ struct IsectParams {
 };
 
 // kernel launchers (mfg_trace.hip)
-int mfg_launch_trace(const TraceParams& p, uint32_t features, int grid, int block, void* stream);
+int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream);
 int mfg_launch_intersect(const IsectParams& p, void* stream);
 int mfg_launch_zero(void* ptr, uint64_t bytes, void* stream);
 int mfg_launch_backward(const double* jac, const double* up, double* grad, uint32_t n_targets, uint32_t n_groups,
                         double scale, void* stream);
 uint32_t mfg_trace_kernel_features(uint32_t model_features);
-int mfg_trace_occupancy(uint32_t features, int block, size_t lds_bytes);
+int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds_bytes);
+int mfg_trace_variants(uint32_t features, int* waves_out, int cap);

@@ -123,6 +123,26 @@ __device__ double interp_y(double x, XYRef t, bool logx, bool logy, bool extrapo
     return logy ? pow(10.0, y) : y;
 }
 
+// interp_x with a guide table: guide[k] is the last point whose y is at or
+// below k/K of the table's last y, so a lookup starts one or two points
+// before the answer; the interval found is the binary search's.
+__device__ double interp_x_guided(double y, XYRef t, const uint32_t* guide, uint32_t K) {
+    uint32_t n = t.n;
+    uint32_t i;
+    double ylast = t.y[n - 1];
+    if (y >= ylast) {
+        i = n - 2;
+    } else {
+        uint32_t k = (uint32_t)fmax(0.0, fmin((double)(K - 1), y / ylast * (double)K));
+        i = guide[k];
+        while (i > 0 && t.y[i] > y) --i;
+        while (i + 1 < n - 1 && t.y[i + 1] <= y) ++i;
+    }
+    double y0 = t.y[i], y1 = t.y[i + 1], x0 = t.x[i], x1 = t.x[i + 1];
+    if (y1 == y0) return x0;
+    return x0 + (y - y0) * (x1 - x0) / (y1 - y0);
+}
+
 __device__ double interp_x(double y, XYRef t) {
     uint32_t n = t.n;
     if (n == 1) return t.x[0];
@@ -319,6 +339,10 @@ struct Tracer {
     uint32_t acc_leak, acc_decay;
     uint32_t tr_n;
     uint32_t* adj_cnt;
+    // pending slot-0 facet counter update of this lane
+    int32_t pend_f;
+    uint32_t pend_hits;
+    double pend_he, pend_abs, pend_s1v, pend_svo, pend_s1pv;
 
     __device__ Tracer(const TraceParams& p, int32_t* s, uint32_t st) : P(p), stk(s), stride(st) {
         acc_hits = acc_des = 0;
@@ -327,6 +351,7 @@ struct Tracer {
         tr_n = 0;
         adj_cnt = nullptr;
         var = nullptr;
+        clear_pending();
     }
 
     __device__ int32_t local(int32_t f) const { return f - (int32_t)var->facet_base; }
@@ -378,23 +403,40 @@ struct Tracer {
         return P.fc + var->fc_base + (size_t)slot * var->n_facets + (uint32_t)ph.local_id;
     }
 
-    // IncreaseFacetCounter (Particle.cpp:1193-1230), slot 0 and slot m.
+    // IncreaseFacetCounter (Particle.cpp:1193-1230). Slot 0 goes to the
+    // lane's pending record (summed across the wave before the atomics);
+    // moment slots and force terms go straight to memory.
     __device__ void counter(const DPhys& ph, int m, uint32_t hit, uint32_t des, uint32_t abs_, double s1v, double svo,
                             d3 imp, d3 imp2, d3 impm) {
         double he = (double)hit * ori;
+        double s1pv = (hit || des) ? (he + (double)des) / vel : 0.0;
+        bool force = false;
+        if constexpr ((FEAT & FEAT_FORCE) != 0) force = var->enable_force != 0;
+        int32_t gf = ph.local_id + (int32_t)var->facet_base;
+        if (!force && des == 0) {
+            if (pend_f >= 0 && pend_f != gf) flush_pending_direct();
+            pend_f = gf;
+            pend_hits += hit;
+            pend_he += he;
+            pend_abs += (double)abs_ * ori;
+            pend_s1v += ori * s1v;
+            pend_svo += ori * svo;
+            pend_s1pv += s1pv;
+        }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             int slot = k == 0 ? 0 : m;
             if (k == 1 && m <= 0) break;
+            if (k == 0 && !force && des == 0) continue;
             mfg_facet_counters* c = fc(ph, slot);
             if (hit) { atomic_addu(&c->nb_mc_hit, hit); atomic_addd(&c->nb_hit_equiv, he); }
             if (des) atomic_addu(&c->nb_desorbed, des);
             if (abs_) atomic_addd(&c->nb_abs_equiv, (double)abs_ * ori);
             if (s1v != 0.0) atomic_addd(&c->sum_1_per_ort_velocity, ori * s1v);
             if (svo != 0.0) atomic_addd(&c->sum_v_ort, ori * svo);
-            if (hit || des) atomic_addd(&c->sum_1_per_velocity, (he + (double)des) / vel);
+            if (hit || des) atomic_addd(&c->sum_1_per_velocity, s1pv);
             if constexpr ((FEAT & FEAT_FORCE) != 0) {
-                if (var->enable_force) {
+                if (force) {
                     double* fo = P.force + var->force_base + ((size_t)slot * var->n_facets + (uint32_t)ph.local_id) * 9;
                     atomic_addd(fo + 0, ori * imp.x); atomic_addd(fo + 1, ori * imp.y); atomic_addd(fo + 2, ori * imp.z);
                     atomic_addd(fo + 3, ori * imp2.x); atomic_addd(fo + 4, ori * imp2.y); atomic_addd(fo + 5, ori * imp2.z);
@@ -406,6 +448,22 @@ struct Tracer {
             if (hit && ph.adj_target >= 0) adjoint_score(ph, he, false);
         }
         (void)imp; (void)imp2; (void)impm;
+    }
+    // Atomics of this lane's pending slot-0 record, then clear it.
+    __device__ void flush_pending_direct() {
+        if (pend_f < 0) return;
+        const DPhys& ph = P.phys[pend_f];
+        mfg_facet_counters* cc = P.fc + var->fc_base + (uint32_t)ph.local_id;
+        if (pend_hits) { atomic_addu(&cc->nb_mc_hit, pend_hits); atomic_addd(&cc->nb_hit_equiv, pend_he); }
+        if (pend_abs != 0.0) atomic_addd(&cc->nb_abs_equiv, pend_abs);
+        if (pend_s1v != 0.0) atomic_addd(&cc->sum_1_per_ort_velocity, pend_s1v);
+        if (pend_svo != 0.0) atomic_addd(&cc->sum_v_ort, pend_svo);
+        if (pend_s1pv != 0.0) atomic_addd(&cc->sum_1_per_velocity, pend_s1pv);
+        clear_pending();
+    }
+    __device__ void clear_pending() {
+        pend_f = -1; pend_hits = 0;
+        pend_he = pend_abs = pend_s1v = pend_svo = pend_s1pv = 0.0;
     }
     // Path score of a hit-equivalent x on a target:
     // d x / d s_g = x * (-n_g / (1 - s_g)) (+ x / s_t for the absorbed part).
@@ -524,11 +582,13 @@ struct Tracer {
         th = acos(fmax(-1.0, fmin(1.0, ln)));
         phv = atan2(lv, lu);
     }
-    // RecordAngleMap (Particle.cpp:1104-1136)
+    // RecordAngleMap (Particle.cpp:1104-1136). The angles only select a bin,
+    // so they are evaluated in FP32.
     __device__ void angle_map(const DFacet* F, const DPhys& ph) {
         if (!(ph.flags & FF_ANGLEMAP) || ph.am_off < 0) return;
-        double th, phv;
-        polar(F, ph, th, phv);
+        double lu = dot(dir, ld3(ph.nU)), lv = dot(dir, ld3(ph.nV)), ln = dot(dir, ld3(F->N));
+        double th = (double)acosf((float)fmax(-1.0, fmin(1.0, ln)));
+        double phv = (double)atan2f((float)lv, (float)lu);
         if (th > kPi / 2.0) th = fabs(kPi - th);
         long ti;
         if (th < ph.am_theta_limit) {
@@ -610,7 +670,7 @@ struct Tracer {
     }
     __device__ double maxwell_speed(double sq) {
         XYRef t{P.tab_x + var->maxwell_base, P.tab_y + var->maxwell_base, 100};
-        return interp_x(rng.rnd(), t) * sq;
+        return interp_x_guided(rng.rnd(), t, P.guides + var->maxwell_guide, MFG_MAXWELL_GUIDE) * sq;
     }
     __device__ d3 emit(const DFacet* F, const DPhys& ph, int type, double n_exp, bool reverse) {
         if (type == MFG_DES_UNIFORM) {
@@ -846,10 +906,8 @@ struct Tracer {
         double dn = dot(dir, N);
         bool revert = (F->flags & FF_2SIDED) && dn > 0.0;
         double ov_in = vel * fabs(dn);
-        double v_in = vel;
         int m = moment(time);
-        bool simple = (FEAT & (FEAT_FORCE | FEAT_ADJOINT | FEAT_SOJOURN)) == 0;
-        if (!simple) counter_force(ph, m, 1, 0, 0, 1.0 / ov_in, var->c_factor * ov_in, 1.0);
+        counter_force(ph, m, 1, 0, 0, 1.0 / ov_in, var->c_factor * ov_in, 1.0);
         ++nb_bounces;
         if ((ph.flags & FF_COUNT_REFL) && ph.tex_off >= 0) texture(F, ph, m, true, 1.0, 1.0);
         if (ph.flags & FF_COUNT_DIR) direction(ph, m);
@@ -884,21 +942,7 @@ struct Tracer {
             if ((ph.flags & FF_MOVING) && var->motion_type) moving(dir, vel);
         }
         double ov_out = vel * ort(F);
-        if (simple) {
-            // incoming and outgoing counters of the same slot in one pass
-            for (int k = 0; k < 2; ++k) {
-                int slot = k == 0 ? 0 : m;
-                if (k == 1 && m <= 0) break;
-                mfg_facet_counters* c = fc(ph, slot);
-                atomic_addu(&c->nb_mc_hit, 1ull);
-                atomic_addd(&c->nb_hit_equiv, ori);
-                atomic_addd(&c->sum_1_per_ort_velocity, ori * (1.0 / ov_in + 1.0 / ov_out));
-                atomic_addd(&c->sum_v_ort, ori * var->c_factor * (ov_in + ov_out));
-                atomic_addd(&c->sum_1_per_velocity, ori / v_in);
-            }
-        } else {
-            counter_force(ph, m_out, 0, 0, 0, 1.0 / ov_out, var->c_factor * ov_out, -1.0);
-        }
+        counter_force(ph, m_out, 0, 0, 0, 1.0 / ov_out, var->c_factor * ov_out, -1.0);
         if ((ph.flags & FF_COUNT_REFL) && ph.tex_off >= 0) texture(F, ph, m_out, false, 1.0, 1.0);
         profile(F, ph, m_out, false, 1.0, 1.0);
         last_hit = f;
@@ -1097,7 +1141,7 @@ __device__ __forceinline__ uint32_t find_variant(const uint64_t* prefix, uint32_
 }
 
 template <uint32_t FEAT>
-__global__ void __launch_bounds__(256) k_trace(TraceParams P) {
+__device__ __forceinline__ void trace_body(const TraceParams& P) {
     extern __shared__ int32_t lds_stack[];
     const uint32_t lane = __lane_id();
     int32_t* stk = lds_stack + threadIdx.x;
@@ -1177,6 +1221,28 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
             alive = T.step();
             if (alive && --events_left == 0) alive = false;
         }
+        // slot-0 facet counters of this iteration: lanes that updated the
+        // same facet as the first pending lane add through one set of atomics
+        uint64_t pm = __ballot(T.pend_f >= 0);
+        if (pm) {
+            uint32_t ld = (uint32_t)__ffsll((unsigned long long)pm) - 1;
+            int32_t key = __shfl(T.pend_f, ld, 64);
+            bool grp = T.pend_f == key;
+            uint64_t gm = __ballot(grp);
+            if (__popcll(gm) > 1) {
+                double he = wave_sum(grp ? T.pend_he : 0.0), ab = wave_sum(grp ? T.pend_abs : 0.0);
+                double s1 = wave_sum(grp ? T.pend_s1v : 0.0), sv = wave_sum(grp ? T.pend_svo : 0.0);
+                double sp = wave_sum(grp ? T.pend_s1pv : 0.0);
+                unsigned long long hc = wave_sum_u(grp ? (unsigned long long)T.pend_hits : 0ull);
+                if (lane == ld) {
+                    T.pend_hits = (uint32_t)hc; T.pend_he = he; T.pend_abs = ab;
+                    T.pend_s1v = s1; T.pend_svo = sv; T.pend_s1pv = sp;
+                } else if (grp) {
+                    T.clear_pending();
+                }
+            }
+            if (T.pend_f >= 0) T.flush_pending_direct();
+        }
     }
     // flush lane accumulators: one set of atomics per wave for a single
     // variant, per lane otherwise
@@ -1199,6 +1265,14 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
     } else {
         T.flush_lane();
     }
+}
+
+// One instantiation per occupancy bound: WAVES waves per SIMD caps the
+// VGPR budget (cold per-particle state spills to scratch at the higher
+// bounds); the host autotuner picks the fastest for a model.
+template <uint32_t FEAT, int WAVES>
+__global__ void __launch_bounds__(256, WAVES) k_trace(TraceParams P) {
+    trace_body<FEAT>(P);
 }
 
 __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
@@ -1244,25 +1318,40 @@ uint32_t mfg_trace_kernel_features(uint32_t model_features) {
     return kLean;
 }
 
-int mfg_trace_occupancy(uint32_t features, int block, size_t lds) {
+namespace {
+typedef void (*TraceKernel)(TraceParams);
+// [feature set][waves per SIMD - 1]; null where not instantiated
+TraceKernel kernel_for(uint32_t features, int waves) {
+    static TraceKernel lean[4] = {nullptr, k_trace<kLean, 2>, k_trace<kLean, 3>, k_trace<kLean, 4>};
+    static TraceKernel full[4] = {k_trace<kFull, 1>, k_trace<kFull, 2>, k_trace<kFull, 3>, nullptr};
+    static TraceKernel adj[4] = {k_trace<kFullAdj, 1>, k_trace<kFullAdj, 2>, nullptr, nullptr};
+    TraceKernel* t = features == kLean ? lean : (features == kFull ? full : adj);
+    if (waves < 1) waves = 1;
+    if (waves > 4) waves = 4;
+    for (int w = waves; w >= 1; --w) if (t[w - 1]) return t[w - 1];
+    for (int w = waves; w <= 4; ++w) if (t[w - 1]) return t[w - 1];
+    return t[0];
+}
+}  // namespace
+
+int mfg_trace_variants(uint32_t features, int* waves_out, int cap) {
+    uint32_t f = mfg_trace_kernel_features(features);
+    int lo = f == kLean ? 2 : 1, hi = f == kLean ? 4 : (f == kFull ? 3 : 2);
     int n = 0;
-    hipError_t e;
-    switch (mfg_trace_kernel_features(features)) {
-        case kLean: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kLean>, block, lds); break;
-        case kFull: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kFull>, block, lds); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<kFullAdj>, block, lds); break;
-    }
+    for (int w = lo; w <= hi && n < cap; ++w) waves_out[n++] = w;
+    return n;
+}
+
+int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds) {
+    int n = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel_for(mfg_trace_kernel_features(features), waves), block, lds);
     return e == hipSuccess ? n : 1;
 }
 
-int mfg_launch_trace(const TraceParams& p, uint32_t features, int grid, int block, void* stream) {
+int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream) {
     size_t lds = (size_t)block * p.stack_size * sizeof(int32_t);
-    hipStream_t s = (hipStream_t)stream;
-    switch (mfg_trace_kernel_features(features)) {
-        case kLean: hipLaunchKernelGGL(k_trace<kLean>, dim3(grid), dim3(block), lds, s, p); break;
-        case kFull: hipLaunchKernelGGL(k_trace<kFull>, dim3(grid), dim3(block), lds, s, p); break;
-        default: hipLaunchKernelGGL(k_trace<kFullAdj>, dim3(grid), dim3(block), lds, s, p); break;
-    }
+    TraceKernel k = kernel_for(mfg_trace_kernel_features(features), waves);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -111,6 +111,14 @@ class GpuSimulation:
     def set_launch(self, block: int = 256, blocks_per_cu: int = 0) -> None:
         self._check(self._lib.mfg_set_launch(self.h, block, blocks_per_cu), "mfg_set_launch")
 
+    def set_kernel_variant(self, waves: int = 0) -> None:
+        self._check(self._lib.mfg_set_kernel_variant(self.h, waves), "mfg_set_kernel_variant")
+
+    def autotune(self, n_probe: int = 100_000) -> A.TuneResult:
+        r = A.TuneResult()
+        self._check(self._lib.mfg_autotune(self.h, int(n_probe), C.byref(r)), "mfg_autotune")
+        return r
+
     def enable_graph(self, on: bool = True) -> None:
         self._check(self._lib.mfg_enable_graph(self.h, int(on)), "mfg_enable_graph")
 

@@ -134,8 +134,19 @@ def test_beam_pipe_absorption_chi2(gpu):
     g = gpu.read()
     n_c = 20_000
     c = CpuModel(m).run(n_c, seed=99)
-    chi2, dof, p = absorption_chi2(g.facets[0]["nb_abs_equiv"], n_g, c.facets[0]["nb_abs_equiv"], n_c)
-    assert dof > 10 and p > 1e-3, (chi2, dof)
+    # absorbers grouped into the 8 pumps and the 6 NEG bands: a 14-way
+    # multinomial (each particle is absorbed exactly once)
+    grp = np.full(m.n_facets, -1)
+    grp[m.pump_ids] = np.arange(len(m.pump_ids))
+    band = np.floor((m.wall_z - 20.0) / 25.0).astype(int)
+    grp[m.neg_mask] = 8 + np.clip(band[m.neg_mask], 0, 5) + (m.wall_z[m.neg_mask] > 100) * 0
+    zz = m.wall_z[m.neg_mask]
+    starts = np.array([20.0, 45.0, 70.0, 120.0, 145.0, 170.0])
+    grp[np.nonzero(m.neg_mask)[0]] = 8 + np.searchsorted(starts, zz, side="right") - 1
+    ag = np.bincount(grp[grp >= 0], weights=g.facets[0]["nb_abs_equiv"][grp >= 0], minlength=14)
+    ac = np.bincount(grp[grp >= 0], weights=c.facets[0]["nb_abs_equiv"][grp >= 0], minlength=14)
+    chi2, dof, p = absorption_chi2(ag, n_g, ac, n_c)
+    assert dof >= 12 and p > 1e-3, (chi2, dof)
     # hits per desorption: batch-means on the GPU side
     hd = g.glob[0]["nb_mc_hit"] / n_g
     hc = c.glob[0]["nb_mc_hit"] / n_c
