@@ -362,6 +362,23 @@ int mfg_read_variant_tallies(mfg_context* ctx, uint32_t variant, const mfg_tally
 /* Launch shape used by runs: threads per block and blocks per CU
  * (0 = defaults / autotuned). */
 int mfg_set_launch(mfg_context* ctx, uint32_t block_threads, uint32_t blocks_per_cu);
+/* Particles kept in flight between runs (the particle state that persists
+ * between SimulationMCStep calls, reference Particle.h:95-122 and
+ * MolflowSimulation.cpp SimulationMCStep loop).
+ * enable = 1: when a run's desorption queue is empty, the particles still in
+ *   flight are parked on the device and the next run resumes them before
+ *   desorbing new ones, so a run never waits for the longest trajectories.
+ *   Tallies then hold every event traced so far; the counts of a resumed
+ *   particle continue its own random stream, so results do not depend on how
+ *   the work is split into runs.
+ * enable = 0 (default): every run traces all its particles, and all parked
+ *   ones, to their end. A run of 0 desorptions with carry disabled drains.
+ * Not available together with the adjoint (MFG_ERR_STATE). mfg_reset and
+ * mfg_load_* drop parked particles. */
+int mfg_set_carry(mfg_context* ctx, int enable);
+/* Number of particles parked on the device after the last run. */
+int mfg_in_flight(const mfg_context* ctx, uint64_t* n);
+
 /* Occupancy bound of the trace kernel: waves per SIMD (1..4; 0 = default).
  * Higher bounds cap registers (cold state spills) to hide more latency. */
 int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);

@@ -155,6 +155,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mfg_set_launch": (C.c_int, [vp, u32, u32]),
         "mfg_enable_graph": (C.c_int, [vp, C.c_int]),
         "mfg_set_kernel_variant": (C.c_int, [vp, u32]),
+        "mfg_set_carry": (C.c_int, [vp, C.c_int]),
+        "mfg_in_flight": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
         "mfg_autotune": (C.c_int, [vp, u64, C.POINTER(TuneResult)]),
         "mfg_adjoint_setup": (C.c_int, [vp, C.POINTER(i32), u32, C.POINTER(i32), u32]),
         "mfg_adjoint_set_sticking": (C.c_int, [vp, dp, u32]),
@@ -174,6 +176,6 @@ EXPORTED_SYMBOLS = (
     "mfg_hits_size", "mfg_intersect_batch", "mfg_maxwell_cdf", "mfg_parse_user_moment",
     "mfg_integrated_desorption", "mfg_set_trace", "mfg_get_trace", "mfg_load_variants", "mfg_run_variants",
     "mfg_get_variant_layout", "mfg_read_variant_tallies", "mfg_set_launch", "mfg_enable_graph",
-    "mfg_set_kernel_variant", "mfg_autotune",
+    "mfg_set_kernel_variant", "mfg_autotune", "mfg_set_carry", "mfg_in_flight",
     "mfg_adjoint_setup", "mfg_adjoint_set_sticking", "mfg_adjoint_read", "mfg_adjoint_backward",
 )

@@ -656,6 +656,10 @@ struct mfg_context {
     int n_cu = 0;
     uint32_t block = 256, blocks_per_cu = 0;
     int waves = 0;                       // occupancy bound of the trace kernel, 0 = default
+    bool carry_on = false;               // keep particles in flight between launches
+    DevBuf<CarryRec> carry;              // two halves of carry_cap records
+    uint64_t carry_cap = 0;
+    uint64_t in_flight = 0;              // particles resident in the carry buffer
     std::vector<std::unique_ptr<HostModel>> models;
     std::vector<VariantLayout> layouts;
     std::vector<DVariant> dvars;
@@ -681,7 +685,7 @@ struct mfg_context {
     DevBuf<mfg_texture_cell> tex;
     DevBuf<mfg_direction_cell> dir;
     DevBuf<mfg_profile_slice> prof;
-    DevBuf<unsigned long long> am, counters;   // counters: next_particle, log_count, leak_count
+    DevBuf<unsigned long long> am, counters;   // counters: next_particle, log_count, leak_count, -, carry control[3]
     DevBuf<DGlobal> glob;
     DevBuf<mfg_log_entry> log;
     DevBuf<double> leaks;
@@ -849,7 +853,7 @@ void upload_models(mfg_context* c) {
     c->prof.alloc(std::max<uint64_t>(prof, 1)); c->am.alloc(std::max<uint64_t>(am, 1));
     c->gh.alloc(std::max<uint64_t>(gh, 1)); c->fh.alloc(std::max<uint64_t>(fh, 1));
     c->glob.alloc(c->models.size());
-    c->counters.alloc(4);
+    c->counters.alloc(8);
     c->log_cap = log_cap;
     c->log.alloc(std::max<uint32_t>(log_cap, 1));
     c->leaks.alloc((size_t)kLeakCap * 6);
@@ -880,6 +884,7 @@ void zero_tallies(mfg_context* c) {
     if (c->adj_j0.p) { z(c->adj_j0.p, c->adj_j0.bytes()); z(c->adj_jac.p, c->adj_jac.bytes()); }
     HIPCHK(hipStreamSynchronize(s));
     c->adj_des = 0;
+    c->in_flight = 0;   // the counters above include the carry control words
 }
 
 void build_accel(mfg_context* c, int width, int leaf) {
@@ -973,6 +978,8 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.total = total;
     p.max_events = max_events;
     p.stop_flag = c->stop_dev;
+    p.carry = c->carry.p; p.carry_ctl = c->counters.p + 4; p.carry_cap = c->carry_cap;
+    p.carry_keep = c->carry_on ? 1u : 0u;
     p.trace_facet = c->trace_f.p; p.trace_kind = c->trace_kind.p; p.trace_n = c->trace_n; p.trace_k = c->trace_k;
     p.adj_j0 = c->adj_j0.p; p.adj_jac = c->adj_jac.p; p.adj_sticking = c->adj_s.p; p.adj_counts = c->adj_counts.p;
     p.adj_groups = c->adj_groups; p.adj_targets = c->adj_targets;
@@ -998,6 +1005,26 @@ void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& wa
     grid = (int)(per_cu * (uint32_t)c->n_cu);
 }
 
+// The carry buffer holds at most one particle per lane of a launch. Growing
+// it keeps the particles already parked in the half the next launch reads.
+void ensure_carry(mfg_context* c, uint64_t lanes) {
+    if (c->carry_cap >= lanes) return;
+    uint64_t cap = std::max<uint64_t>(lanes, (uint64_t)c->n_cu * 2048);
+    DevBuf<CarryRec> nb;
+    nb.alloc(2 * cap);
+    unsigned long long ctl[3] = {0, 0, 0};
+    if (c->carry.p) {
+        HIPCHK(hipMemcpy(ctl, c->counters.p + 4, sizeof(ctl), hipMemcpyDeviceToHost));
+        uint64_t half = ctl[2] & 1ull;
+        if (ctl[half]) HIPCHK(hipMemcpy(nb.p + half * cap, c->carry.p + half * c->carry_cap, ctl[half] * sizeof(CarryRec),
+                                        hipMemcpyDeviceToDevice));
+    }
+    std::swap(c->carry.p, nb.p);
+    std::swap(c->carry.n, nb.n);
+    c->carry_cap = cap;
+    reset_graph(c);
+}
+
 void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, uint64_t max_events, mfg_step_result* res) {
     if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
     if (!c->accel) throw Err(MFG_ERR_STATE, "acceleration structure not built (call mfg_build_accel)");
@@ -1017,9 +1044,11 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     int grid, block, waves;
     launch_shape(c, feats, grid, block, waves);
     if (c->adj_groups) {
+        if (c->carry_on || c->in_flight) throw Err(MFG_ERR_STATE, "the adjoint traces every particle to its end: disable carry and drain first");
         size_t need = (size_t)grid * block * c->adj_groups;
         if (c->adj_counts.n < need) c->adj_counts.alloc(need);
     }
+    if (c->carry_on) ensure_carry(c, (uint64_t)grid * (uint64_t)block);
     // read the global counters before the run to report this run's hits
     std::vector<DGlobal> before(nv);
     hipStream_t s = c->stream;
@@ -1029,6 +1058,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     std::vector<uint64_t> key(n_des, n_des + nv);
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
     key.push_back((uint64_t)waves);
+    key.push_back((uint64_t)(uintptr_t)p.carry); key.push_back(p.carry_keep);
     // the events bracket the trace kernel alone (eager path) or the whole
     // graph (replay path)
     auto enqueue = [&](bool events) {
@@ -1037,6 +1067,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
         if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s)) throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
         if (events) HIPCHK(hipEventRecord(c->ev0, s));
         if (mfg_launch_trace(p, feats, waves, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
+        if (p.carry && mfg_launch_carry_flip(p.carry_ctl, s)) throw Err(MFG_ERR_DEVICE, "carry flip launch failed");
     };
     if (c->graph_on) {
         HIPCHK(hipEventRecord(c->ev0, s));
@@ -1058,6 +1089,11 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     std::vector<DGlobal> after(nv);
     HIPCHK(hipMemcpy(after.data(), c->glob.p, nv * sizeof(DGlobal), hipMemcpyDeviceToHost));
+    if (p.carry) {
+        unsigned long long ctl[3];
+        HIPCHK(hipMemcpy(ctl, c->counters.p + 4, sizeof(ctl), hipMemcpyDeviceToHost));
+        c->in_flight = ctl[ctl[2] & 1ull];
+    }
     uint64_t hits = 0, des = 0;
     for (size_t v = 0; v < nv; ++v) {
         hits += after[v].nb_mc_hit - before[v].nb_mc_hit;
@@ -1422,6 +1458,21 @@ int mfg_set_launch(mfg_context* c, uint32_t block, uint32_t per_cu) {
         c->block = block ? block : 256;
         c->blocks_per_cu = per_cu;
         reset_graph(c);
+    });
+}
+
+int mfg_set_carry(mfg_context* c, int enable) {
+    return guarded(c, [&] {
+        if (enable && c->adj_groups) throw Err(MFG_ERR_STATE, "the adjoint traces every particle to its end (no carry)");
+        c->carry_on = enable != 0;
+        reset_graph(c);
+    });
+}
+
+int mfg_in_flight(const mfg_context* c, uint64_t* n) {
+    return guarded(const_cast<mfg_context*>(c), [&] {
+        if (!n) throw Err(MFG_ERR_INVALID, "null output");
+        *n = c->in_flight;
     });
 }
 

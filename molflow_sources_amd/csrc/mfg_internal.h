@@ -173,6 +173,19 @@ struct DTable {                        // a (x,y) table in the global table pool
 };
 
 // Everything a trace launch needs; passed by value (kernarg segment).
+// One particle still in flight at the end of a launch, kept on the device
+// and resumed first by the next launch (the particle state that persists
+// between the reference's SimulationMCStep calls, Particle.h:95-122).
+struct CarryRec {
+    double pos[3], dir[3];
+    double time, vel, ori, gen_time, dist, decay_t, scatter_path, hu, hv;
+    uint64_t pid, events_left;
+    uint32_t k0, k1, blk, b0, b1, b2, b3, avail;   // random stream position
+    int32_t structure, last_hit, tele_from;
+    uint32_t nb_bounces, nb_events, variant, tr_n;
+};
+static_assert(sizeof(CarryRec) == 200, "CarryRec layout");
+
 struct TraceParams {
     const DFacet* facets;
     const DPhys* phys;
@@ -215,6 +228,13 @@ struct TraceParams {
     uint64_t total;                    // desorptions in this run
     uint64_t max_events;
     volatile int32_t* stop_flag;       // host-mapped
+    // particles kept between launches: two halves of carry_cap records;
+    // carry_ctl[0..1] count the records of each half, carry_ctl[2] names the
+    // half this launch resumes from (the other half receives its leftovers)
+    CarryRec* carry;
+    unsigned long long* carry_ctl;
+    uint64_t carry_cap;
+    uint32_t carry_keep;               // 1: leave particles in flight when the queue runs dry
     // trace
     int32_t* trace_facet;
     int32_t* trace_kind;
@@ -248,6 +268,7 @@ struct IsectParams {
 // kernel launchers (mfg_trace.hip)
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream);
 int mfg_launch_intersect(const IsectParams& p, void* stream);
+int mfg_launch_carry_flip(unsigned long long* carry_ctl, void* stream);
 int mfg_launch_zero(void* ptr, uint64_t bytes, void* stream);
 int mfg_launch_backward(const double* jac, const double* up, double* grad, uint32_t n_targets, uint32_t n_groups,
                         double scale, void* stream);

@@ -1006,6 +1006,31 @@ struct Tracer {
     }
 
     // Add this lane's global counters to its variant and clear them.
+    // ---- particles kept between launches
+    __device__ void save(CarryRec& r, uint64_t events_left, uint32_t vi) const {
+        r.pos[0] = pos.x; r.pos[1] = pos.y; r.pos[2] = pos.z;
+        r.dir[0] = dir.x; r.dir[1] = dir.y; r.dir[2] = dir.z;
+        r.time = time; r.vel = vel; r.ori = ori; r.gen_time = gen_time; r.dist = dist;
+        r.decay_t = decay_t; r.scatter_path = scatter_path; r.hu = hu; r.hv = hv;
+        r.pid = pid; r.events_left = events_left;
+        r.k0 = rng.k0; r.k1 = rng.k1; r.blk = rng.blk;
+        r.b0 = rng.b0; r.b1 = rng.b1; r.b2 = rng.b2; r.b3 = rng.b3; r.avail = rng.avail;
+        r.structure = structure; r.last_hit = last_hit; r.tele_from = tele_from;
+        r.nb_bounces = nb_bounces; r.nb_events = nb_events; r.variant = vi; r.tr_n = tr_n;
+    }
+    __device__ uint64_t restore(const CarryRec& r) {
+        pos = mk(r.pos[0], r.pos[1], r.pos[2]);
+        dir = mk(r.dir[0], r.dir[1], r.dir[2]);
+        time = r.time; vel = r.vel; ori = r.ori; gen_time = r.gen_time; dist = r.dist;
+        decay_t = r.decay_t; scatter_path = r.scatter_path; hu = r.hu; hv = r.hv;
+        pid = r.pid;
+        rng.k0 = r.k0; rng.k1 = r.k1; rng.p0 = (uint32_t)r.pid; rng.p1 = (uint32_t)(r.pid >> 32);
+        rng.blk = r.blk; rng.b0 = r.b0; rng.b1 = r.b1; rng.b2 = r.b2; rng.b3 = r.b3; rng.avail = r.avail;
+        structure = r.structure; last_hit = r.last_hit; tele_from = r.tele_from;
+        nb_bounces = r.nb_bounces; nb_events = r.nb_events; tr_n = r.tr_n;
+        return r.events_left;
+    }
+
     __device__ void flush_lane() {
         if (var == nullptr) return;
         DGlobal* gl = P.glob + var->glob_index;
@@ -1152,28 +1177,56 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     bool alive = false, done = false;
     uint32_t vi = 0;
     uint64_t events_left = 0;
+    // work items: first the particles the previous launch left in flight,
+    // then this launch's desorptions
+    uint64_t n_in = 0;
+    const CarryRec* carry_in = nullptr;
+    CarryRec* carry_out = nullptr;
+    unsigned long long* n_out = nullptr;
+    if (P.carry) {
+        uint32_t half = (uint32_t)P.carry_ctl[2] & 1u;
+        n_in = P.carry_ctl[half];
+        carry_in = P.carry + (size_t)half * P.carry_cap;
+        carry_out = P.carry + (size_t)(half ^ 1u) * P.carry_cap;
+        n_out = P.carry_ctl + (half ^ 1u);
+    }
+    const uint64_t items = n_in + P.total;
     for (;;) {
         bool need = !alive && !done;
         uint64_t mask = __ballot(need);
+        bool dry = false;   // some lane of this wave found the queue exhausted
         if (mask) {
             uint32_t cnt = (uint32_t)__popcll(mask);
             uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
             unsigned long long base = 0;
+            int stop = 0;
             if (lane == leader) {
-                int stop = *P.stop_flag;
-                base = stop ? P.total : atomicAdd(P.next_particle, (unsigned long long)cnt);
+                stop = *P.stop_flag;
+                base = atomicAdd(P.next_particle, (unsigned long long)cnt);
             }
             base = __shfl(base, leader, 64);
+            stop = __shfl(stop, leader, 64);
+            // after a stop request no new particle starts; resumed ones still
+            // run (to the end, or back into the carry buffer)
+            const uint64_t limit = stop ? n_in : items;
             bool started = false;
             int32_t src = -1;
             if (need) {
                 uint64_t g = base + mbcnt(mask);
-                if (g < P.total) {
-                    vi = P.n_variants > 1 ? find_variant(P.var_prefix, P.n_variants, g) : 0;
+                if (g < n_in) {
+                    const CarryRec& r = carry_in[g];
+                    vi = r.variant;
+                    if (T.var != P.variants + vi) T.flush_lane();
+                    T.var = P.variants + vi;
+                    events_left = T.restore(r);
+                    alive = true;
+                } else if (g < limit) {
+                    uint64_t d = g - n_in;
+                    vi = P.n_variants > 1 ? find_variant(P.var_prefix, P.n_variants, d) : 0;
                     if (T.var != P.variants + vi) T.flush_lane();
                     T.var = P.variants + vi;
                     ++T.acc_des;
-                    uint64_t lp = g - P.var_prefix[vi] + T.var->pid_base;
+                    uint64_t lp = d - P.var_prefix[vi] + T.var->pid_base;
                     src = T.start(lp);
                     started = true;
                     alive = true;
@@ -1182,6 +1235,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
                     done = true;
                 }
             }
+            dry = __ballot(done && need) != 0;
             // desorption counters: lanes sharing the leader's source facet
             // add through one set of atomics
             uint64_t smask = __ballot(started);
@@ -1215,6 +1269,19 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
                 }
                 if (started) T.desorption_records(src, m, ov);
             }
+        }
+        if (P.carry_keep && dry) {
+            // the queue is empty: park this wave's particles for the next
+            // launch instead of running their tails here
+            uint64_t am = __ballot(alive);
+            if (am) {
+                uint32_t al = (uint32_t)__ffsll((unsigned long long)am) - 1;
+                unsigned long long ob = 0;
+                if (lane == al) ob = atomicAdd(n_out, (unsigned long long)__popcll(am));
+                ob = __shfl(ob, al, 64);
+                if (alive) T.save(carry_out[ob + mbcnt(am)], events_left, vi);
+            }
+            break;
         }
         if (!__ballot(!done)) break;
         if (alive) {
@@ -1273,6 +1340,16 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
 template <uint32_t FEAT, int WAVES>
 __global__ void __launch_bounds__(256, WAVES) k_trace(TraceParams P) {
     trace_body<FEAT>(P);
+}
+
+// After a launch: the half it resumed from is consumed, and the half that
+// received its leftovers becomes the next launch's input.
+__global__ void k_carry_flip(unsigned long long* ctl) {
+    if (threadIdx.x == 0) {
+        unsigned long long half = ctl[2] & 1ull;
+        ctl[half] = 0ull;
+        ctl[2] = half ^ 1ull;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
@@ -1352,6 +1429,11 @@ int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int gri
     size_t lds = (size_t)block * p.stack_size * sizeof(int32_t);
     TraceKernel k = kernel_for(mfg_trace_kernel_features(features), waves);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mfg_launch_carry_flip(unsigned long long* ctl, void* stream) {
+    hipLaunchKernelGGL(k_carry_flip, dim3(1), dim3(64), 0, (hipStream_t)stream, ctl);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

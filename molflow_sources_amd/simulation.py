@@ -111,6 +111,22 @@ class GpuSimulation:
     def set_launch(self, block: int = 256, blocks_per_cu: int = 0) -> None:
         self._check(self._lib.mfg_set_launch(self.h, block, blocks_per_cu), "mfg_set_launch")
 
+    def set_carry(self, on: bool = True) -> None:
+        """Keep particles in flight between runs (see mfg_set_carry)."""
+        self._check(self._lib.mfg_set_carry(self.h, int(on)), "mfg_set_carry")
+
+    def in_flight(self) -> int:
+        n = C.c_uint64()
+        self._check(self._lib.mfg_in_flight(self.h, C.byref(n)), "mfg_in_flight")
+        return n.value
+
+    def drain(self, max_events: int = 0) -> A.StepResult:
+        """Trace every parked particle to its end (carry is switched off)."""
+        self.set_carry(False)
+        return self.run(0, max_events=max_events) if self.models is None else \
+            self.run_variants(np.zeros(len(self.models), dtype=np.uint64), np.zeros(len(self.models), dtype=np.uint64),
+                              max_events=max_events)
+
     def set_kernel_variant(self, waves: int = 0) -> None:
         self._check(self._lib.mfg_set_kernel_variant(self.h, waves), "mfg_set_kernel_variant")
 
