@@ -31,30 +31,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per hit (one wall event), from the reference-equivalent
-# BVH (binary SAH, <= 2 facets per leaf): 32 B per visited node (FP32 box +
-# child word), 48 B per tested facet (plane, origin, edge vectors as FP32
-# equivalents), 96 B of the hit facet's parameters, 16 B per tally field
-# updated (read + write of 8 B) with 5 fields per bounce.
-NODE_BYTES, PRIM_BYTES, FACET_BYTES, FIELD_BYTES, FIELDS_PER_HIT = 32, 48, 96, 16, 5
+# Algorithmic bytes per hit (one wall event), counted on the reference-
+# equivalent BVH (binary SAH, <= 2 facets per leaf):
+#   B_hit = 32 N_node + 48 N_prim + 96 + 16 N_field
+# 32 B per visited node (FP32 box + child word), 48 B per tested triangle
+# (plane, origin and edge vectors as FP32 equivalents), 96 B of the hit
+# facet's parameters and 16 B (read + write of 8 B) per tally field updated:
+# 7 counter fields per hit plus 3 texture fields per hit on a textured facet.
+NODE_BYTES, PRIM_BYTES, FACET_BYTES, FIELD_BYTES, FIELDS_PER_HIT, TEX_FIELDS = 32, 48, 96, 16, 7, 3
 
 
-def algorithmic_bytes_per_hit(model, n_rays=20000, seed=5):
-    """Mean BVH nodes and facet tests per ray on the oracle's BVH, from
-    particles' own rays (desorption + one diffuse bounce positions)."""
+def wall_rays(n_rays, seed=5, radius=4.0, length=200.0):
+    """Particle rays of the beam pipe: points on the main wall with cosine
+    (Lambertian) directions about the inward normal, i.e. diffuse bounces."""
     import numpy as np
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(1, length - 1, n_rays)
+    a = rng.uniform(0, 2 * math.pi, n_rays)
+    r = radius * (1 - 1e-4)
+    o = np.stack([r * np.cos(a), r * np.sin(a), z], 1)
+    nrm = -np.stack([np.cos(a), np.sin(a), np.zeros(n_rays)], 1)
+    t1 = np.stack([-np.sin(a), np.cos(a), np.zeros(n_rays)], 1)
+    t2 = np.cross(nrm, t1)
+    u1, u2 = rng.random(n_rays), rng.random(n_rays)
+    ct, st, ph = np.sqrt(1 - u1), np.sqrt(u1), 2 * math.pi * u2
+    d = nrm * ct[:, None] + t1 * (st * np.cos(ph))[:, None] + t2 * (st * np.sin(ph))[:, None]
+    return o, d
 
+
+def algorithmic_bytes_per_hit(model, tex_hit_fraction, n_rays=20000):
+    """Mean BVH nodes and facet tests per ray on the oracle's BVH, over
+    diffuse-bounce rays of the workload."""
     from oracle.mfcpu import CpuModel
     cm = CpuModel(model)
-    rng = np.random.default_rng(seed)
-    z = rng.uniform(1, 199, n_rays)
-    r = rng.uniform(0, 3.9, n_rays)
-    a = rng.uniform(0, 2 * math.pi, n_rays)
-    o = np.stack([r * np.cos(a), r * np.sin(a), z], 1)
-    d = rng.normal(size=(n_rays, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o, d = wall_rays(n_rays)
     nodes, prims = cm.bvh_stats(o, d)
-    b = NODE_BYTES * nodes + PRIM_BYTES * prims + FACET_BYTES + FIELD_BYTES * FIELDS_PER_HIT
+    fields = FIELDS_PER_HIT + TEX_FIELDS * tex_hit_fraction
+    b = NODE_BYTES * nodes + PRIM_BYTES * prims + FACET_BYTES + FIELD_BYTES * fields
     return b, nodes, prims
 
 
@@ -99,6 +112,7 @@ def main():
     ap.add_argument("--des-per-step", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-carry", action="store_true", help="trace every launch's particles to their end")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -129,6 +143,13 @@ def main():
     sim.set_launch(args.block, args.blocks_per_cu)
     sim.set_carry(not args.no_carry)
     sim.reset(seed=42)
+    tune = None
+    if not args.no_tune:
+        # occupancy bound and block size for this model (the tuner discards
+        # its probe runs: it zeroes the tallies afterwards)
+        tr = sim.autotune(args.des_per_step)
+        tune = {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu,
+                "probe_hits_per_s": tr.hits_per_s}
     max_events = 50_000_000
     for _ in range(args.warmup):
         sim.run(args.des_per_step, max_events=max_events)
@@ -152,7 +173,10 @@ def main():
     assert t.glob[0]["nb_desorbed"] == args.steps * args.des_per_step and sim.in_flight() == 0
     assert t.glob[0]["nb_mc_hit"] == hits
     value = hits / wall
-    b_hit, nodes, prims = algorithmic_bytes_per_hit(model)
+    fh = t.facets["nb_mc_hit"][0].astype(float)
+    tex_ids = list(getattr(model, "tex_ids", []))
+    tex_frac = float(fh[tex_ids].sum() / max(fh.sum(), 1.0)) if tex_ids else 0.0
+    b_hit, nodes, prims = algorithmic_bytes_per_hit(model, tex_frac)
     launches = args.steps + 1
     launch_s = kernel_ms / 1e3 / launches
     hits_per_launch = hits / launches
@@ -174,7 +198,7 @@ def main():
                    "facets": model.n_facets, "desorptions_per_step": args.des_per_step,
                    "hits_per_desorption": hits / (args.steps * args.des_per_step),
                    "seed": 42, "parallelism": "single GPU", "carry": not args.no_carry,
-                   "drain_ms": drain_s * 1e3, "job": "steps x desorptions_per_step, every particle traced to its end"},
+                   "drain_ms": drain_s * 1e3, "launch": tune, "job": "steps x desorptions_per_step, every particle traced to its end"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "bytes_per_hit": b_hit, "bvh_nodes_per_ray": nodes, "facet_tests_per_ray": prims,
