@@ -195,6 +195,27 @@ __device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const d
     return in_poly(poly + 2 * (size_t)F->poly_off, F->poly_n, u, v);
 }
 
+// facet_test on a leaf record: the same operations in the same order, with
+// every operand loaded before the first decision so that the loads of one
+// record issue together (one round trip per leaf entry).
+__device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const double* __restrict__ poly, d3 P, d3 D,
+                                          double& t, double& u, double& v) {
+    d3 N = ld3(L->N), O = ld3(L->O), U = ld3(L->U), V = ld3(L->V);
+    double iu = L->invU2, iv = L->invV2;
+    uint32_t flags = L->flags, n = L->poly_n;
+    double denom = fdot(N, D);
+    d3 w = sub(O, P);
+    t = fdot(N, w) / denom;
+    d3 X = mk(fma(t, D.x, P.x), fma(t, D.y, P.y), fma(t, D.z, P.z));
+    d3 r = sub(X, O);
+    u = fdot(r, U) * iu;
+    v = fdot(r, V) * iv;
+    bool ok = denom != 0.0 && ((flags & FF_2SIDED) || !(denom > 0.0)) && t > 0.0 &&
+              !(u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0);
+    if (!ok) return false;
+    return in_poly(n == 3 ? L->uv : poly + 2 * (size_t)L->poly_off, n, u, v);
+}
+
 struct Ray32 { float ox, oy, oz, ix, iy, iz; };
 
 __device__ __forceinline__ float safe_inv(double d) {
@@ -255,33 +276,28 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         const BvhNode* nd = P.nodes + node;
         float tn[4];
         uint32_t m = node_hits(nd, r, tbf, tn);
-        // inner children that survive the (possibly shrunk) tmax, sorted
-        // near-to-far with a fixed network (registers only)
-        constexpr float INF = 3.4e38f;
-        float k0 = INF, k1 = INF, k2 = INF, k3 = INF;
-        int32_t c0 = -1, c1 = -1, c2 = -1, c3 = -1;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (!(m & (1u << c))) continue;
-            int32_t ch = nd->child[c];
-            if (ch >= 0) {
-                if (c == 0) { k0 = tn[0]; c0 = ch; }
-                else if (c == 1) { k1 = tn[1]; c1 = ch; }
-                else if (c == 2) { k2 = tn[2]; c2 = ch; }
-                else { k3 = tn[3]; c3 = ch; }
-                continue;
-            }
-            uint32_t first = (uint32_t)(~ch), cnt = nd->count[c];
+        const int32_t ch0 = nd->child[0], ch1 = nd->child[1], ch2 = nd->child[2], ch3 = nd->child[3];
+        const uint32_t cnts = (uint32_t)nd->count[0] | ((uint32_t)nd->count[1] << 8) |
+                              ((uint32_t)nd->count[2] << 16) | ((uint32_t)nd->count[3] << 24);
+        // leaf children first (one loop for all slots, so lanes whose leaves
+        // sit in different slots test their facets together)
+        uint32_t lm = ((m & 1u) && ch0 < 0 ? 1u : 0u) | ((m & 2u) && ch1 < 0 ? 2u : 0u) |
+                      ((m & 4u) && ch2 < 0 ? 4u : 0u) | ((m & 8u) && ch3 < 0 ? 8u : 0u);
+        while (lm) {
+            uint32_t c = (uint32_t)__builtin_ctz(lm);
+            lm &= lm - 1u;
+            int32_t ch = c == 0 ? ch0 : (c == 1 ? ch1 : (c == 2 ? ch2 : ch3));
+            uint32_t first = (uint32_t)(~ch), cnt = (cnts >> (8u * c)) & 0xffu;
             for (uint32_t k = 0; k < cnt; ++k) {
-                int32_t f = P.leaf_facets[first + k];
-                if (f == skip) continue;
-                const DFacet* F = P.facets + f;
+                const LeafFacet* L = P.leaves + first + k;
                 double t, u, v;
-                if (!facet_test(F, P.poly, Pp, D, t, u, v)) continue;
+                bool hit = leaf_test(L, P.poly, Pp, D, t, u, v);
+                int32_t f = L->id;
+                if (!hit || f == skip) continue;
                 if (!(t < h.t || (t == h.t && f < h.facet))) continue;
                 bool hard = true;
                 if constexpr (TRANSP) {
-                    uint32_t fl = F->flags;
+                    uint32_t fl = L->flags;
                     if (!opaque_only && !(fl & FF_OPAQUE)) {
                         double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
                         if (op < 1.0) hard = (op > 0.0) && rng->coin(event, (uint32_t)(f - (int32_t)var->facet_base)) < op;
@@ -293,6 +309,12 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
                 }
             }
         }
+        // inner children that survive the (possibly shrunk) tmax, sorted
+        // near-to-far with a fixed network (registers only)
+        constexpr float INF = 3.4e38f;
+        const bool i0 = (m & 1u) && ch0 >= 0, i1 = (m & 2u) && ch1 >= 0, i2 = (m & 4u) && ch2 >= 0, i3 = (m & 8u) && ch3 >= 0;
+        float k0 = i0 ? tn[0] : INF, k1 = i1 ? tn[1] : INF, k2 = i2 ? tn[2] : INF, k3 = i3 ? tn[3] : INF;
+        int32_t c0 = i0 ? ch0 : -1, c1 = i1 ? ch1 : -1, c2 = i2 ? ch2 : -1, c3 = i3 ? ch3 : -1;
         if (k0 > tbf) { k0 = INF; c0 = -1; }
         if (k1 > tbf) { k1 = INF; c1 = -1; }
         if (k2 > tbf) { k2 = INF; c2 = -1; }
@@ -1092,13 +1114,13 @@ struct Tracer {
                         }
                         uint32_t first = (uint32_t)(~ch), cnt = nd->count[c];
                         for (uint32_t kk = 0; kk < cnt; ++kk) {
-                            int32_t f = P.leaf_facets[first + kk];
+                            const LeafFacet* L = P.leaves + first + kk;
+                            int32_t f = L->id;
                             if (f == last_hit) continue;
-                            const DFacet* F = P.facets + f;
-                            uint32_t fl = F->flags;
+                            uint32_t fl = L->flags;
                             if (fl & FF_OPAQUE) continue;
                             double t, u, v;
-                            if (!facet_test(F, P.poly, pos, dir, t, u, v)) continue;
+                            if (!leaf_test(L, P.poly, pos, dir, t, u, v)) continue;
                             if (!(t < travel)) continue;
                             double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
                             bool hard = op >= 1.0 || (op > 0.0 && rng.coin(nb_events, (uint32_t)local(f)) < op);
@@ -1357,7 +1379,7 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     TraceParams tp{};
-    tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaf_facets = p.leaf_facets;
+    tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaves = p.leaves;
     d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
     int32_t skip = p.last_hit ? p.last_hit[i] : -1;
     if (skip >= 0) skip += (int32_t)p.facet_base;
