@@ -675,6 +675,7 @@ struct mfg_context {
     DevBuf<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_start, mom_end, tex_inc;
     DevBuf<int32_t> src_facet, roots;
     DevBuf<LeafFacet> leaves;
+    DevBuf<LeafUV> leaf_uv;
     std::vector<DFacet> h_facets;        // host copies of the concatenated facets and polygons
     std::vector<double> h_poly;
     DevBuf<uint32_t> guides;
@@ -962,17 +963,20 @@ void build_accel(mfg_context* c, int width, int leaf) {
     c->nodes.upload(nodes, s);
     // leaf-ordered copies of the test data (each facet sits in one leaf)
     std::vector<LeafFacet> lf(leafs.size());
+    std::vector<LeafUV> luv(leafs.size());
     for (size_t i = 0; i < leafs.size(); ++i) {
         const DFacet& F = c->h_facets.at((size_t)leafs[i]);
         LeafFacet& L = lf[i];
         std::memset(&L, 0, sizeof(L));
+        std::memset(&luv[i], 0, sizeof(LeafUV));
         for (int k = 0; k < 3; ++k) { L.O[k] = F.O[k]; L.N[k] = F.N[k]; L.U[k] = F.U[k]; L.V[k] = F.V[k]; }
         L.invU2 = F.invU2; L.invV2 = F.invV2;
         L.id = leafs[i]; L.flags = F.flags; L.poly_off = F.poly_off; L.poly_n = F.poly_n;
         if (F.poly_n == 3)
-            for (int k = 0; k < 6; ++k) L.uv[k] = c->h_poly[2 * (size_t)F.poly_off + k];
+            for (int k = 0; k < 6; ++k) luv[i].uv[k] = c->h_poly[2 * (size_t)F.poly_off + k];
     }
     c->leaves.upload(lf, s);
+    c->leaf_uv.upload(luv, s);
     c->roots.upload(roots, s);
     HIPCHK(hipStreamSynchronize(s));
     c->accel = true;
@@ -982,7 +986,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
 TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     TraceParams p{};
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
-    p.leaves = c->leaves.p; p.roots = c->roots.p; p.variants = c->variants.p;
+    p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size(); p.stack_size = c->stack_size;
     p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p;
     p.guides = c->guides.p; p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
@@ -1403,7 +1407,7 @@ int mfg_intersect_batch(mfg_context* c, uint64_t n, const double* o, const doubl
         HIPCHK(hipMemcpyAsync(dD.p, d, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
         if (last) { dL.alloc(n); HIPCHK(hipMemcpyAsync(dL.p, last, n * sizeof(int32_t), hipMemcpyHostToDevice, s)); }
         IsectParams p{};
-        p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p;
+        p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p;
         p.root = roots[c->dvars[0].root_base + (uint32_t)structure];
         p.stack_size = c->stack_size; p.n = n; p.origins = dO.p; p.dirs = dD.p; p.last_hit = last ? dL.p : nullptr;
         p.out_facet = dF.p; p.out_t = dT.p; p.out_u = dU.p; p.out_v = dV.p; p.facet_base = c->dvars[0].facet_base;

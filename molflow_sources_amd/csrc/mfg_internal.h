@@ -6,8 +6,9 @@
 //   DPhys[nF_total]       physics and recording parameters, read on a hit
 //   double2 poly[]        normalised (u,v) polygon vertices
 //   BvhNode[]             4-wide nodes, 128 B each, FP32 boxes
-//   LeafFacet[]           the ray/facet test data in leaf order (192 B):
-//                         a leaf is a contiguous run, no index indirection
+//   LeafFacet[]           the ray/facet test data in leaf order (128 B) and
+//   LeafUV[]              triangle (u,v) polygons (64 B): a leaf is a
+//                         contiguous run, no index indirection
 //   DVariant[nV]          per-variant bases and settings
 //   tallies               [variant][slot][facet] 64 B counters, textures,
 //                         profiles, ... (layout of include/molflow_gpu.h)
@@ -118,19 +119,24 @@ struct alignas(16) BvhNode {
 static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 bytes");
 
 // Exact (FP64) ray/facet test data of one leaf entry, stored in leaf order
-// so that a leaf's facets are one contiguous fetch. Triangles carry their
-// (u,v) polygon inline; other polygons point into the shared poly array.
-struct alignas(64) LeafFacet {
-    double O[3], N[3], U[3], V[3];
-    double invU2, invV2;
-    double uv[6];          // (u,v) of a triangle's vertices
+// so that a leaf's facets are contiguous (no index indirection). One record
+// is one 128 B line, loaded as eight 16 B loads issued together.
+struct alignas(128) LeafFacet {
     int32_t id;            // global facet index
     uint32_t flags;        // FF_* of the facet
     uint32_t poly_off;     // first (u,v) pair in poly (polygons of > 3 vertices)
     uint32_t poly_n;
-    uint32_t pad[4];
+    double O[3], N[3], U[3], V[3];
+    double invU2, invV2;
 };
-static_assert(sizeof(LeafFacet) == 192, "LeafFacet must be 192 bytes");
+static_assert(sizeof(LeafFacet) == 128, "LeafFacet must be one 128 B line");
+// (u,v) of a triangle's vertices, parallel to LeafFacet (read only when the
+// ray meets the facet's (u,v) rectangle)
+struct alignas(64) LeafUV {
+    double uv[6];
+    double pad[2];
+};
+static_assert(sizeof(LeafUV) == 64, "LeafUV layout");
 
 // global counters of one variant (mfg_global_counters)
 struct DGlobal {
@@ -208,6 +214,7 @@ struct TraceParams {
     const double* poly;               // (u,v) pairs
     const BvhNode* nodes;
     const LeafFacet* leaves;
+    const LeafUV* leaf_uv;
     const int32_t* roots;             // per (variant, structure): root node or -1
     const DVariant* variants;
     const uint64_t* var_prefix;       // n_variants+1 prefix of this run's desorptions
@@ -268,6 +275,7 @@ struct IsectParams {
     const double* poly;
     const BvhNode* nodes;
     const LeafFacet* leaves;
+    const LeafUV* leaf_uv;
     int32_t root;
     uint32_t stack_size;
     uint64_t n;

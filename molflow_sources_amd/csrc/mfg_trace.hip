@@ -174,6 +174,25 @@ __device__ __forceinline__ bool in_poly(const double* __restrict__ p, uint32_t n
     return inside;
 }
 
+// in_poly for a triangle held in registers: the same crossing tests in the
+// same order as the loop above with n = 3.
+__device__ __forceinline__ bool crossing(double ui, double vi, double uj, double vj, double u, double v) {
+    if ((vi > v) != (vj > v)) {
+        double x = (uj - ui) * (v - vi) / (vj - vi) + ui;
+        return u < x;
+    }
+    return false;
+}
+__device__ __forceinline__ bool in_tri(const LeafUV* __restrict__ LU, double u, double v) {
+    const double2* q = reinterpret_cast<const double2*>(LU->uv);
+    const double2 p0 = q[0], p1 = q[1], p2 = q[2];
+    bool inside = false;
+    if (crossing(p0.x, p0.y, p2.x, p2.y, u, v)) inside = !inside;
+    if (crossing(p1.x, p1.y, p0.x, p0.y, u, v)) inside = !inside;
+    if (crossing(p2.x, p2.y, p1.x, p1.y, u, v)) inside = !inside;
+    return inside;
+}
+
 // Ray/facet test; the same arithmetic, operation for operation, as the CPU
 // restatement (explicit fma dot products, contraction off elsewhere).
 __device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const double* __restrict__ poly, d3 P, d3 D,
@@ -195,14 +214,21 @@ __device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const d
     return in_poly(poly + 2 * (size_t)F->poly_off, F->poly_n, u, v);
 }
 
-// facet_test on a leaf record: the same operations in the same order, with
-// every operand loaded before the first decision so that the loads of one
-// record issue together (one round trip per leaf entry).
-__device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const double* __restrict__ poly, d3 P, d3 D,
-                                          double& t, double& u, double& v) {
-    d3 N = ld3(L->N), O = ld3(L->O), U = ld3(L->U), V = ld3(L->V);
-    double iu = L->invU2, iv = L->invV2;
-    uint32_t flags = L->flags, n = L->poly_n;
+// facet_test on a leaf record: the same operations in the same order. The
+// record's 128 B are loaded together before the first decision (one round
+// trip); a triangle's (u,v) polygon is fetched only when the ray meets the
+// facet's rectangle.
+__device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const LeafUV* __restrict__ LU,
+                                          const double* __restrict__ poly, d3 P, d3 D, int32_t skip, double& t,
+                                          double& u, double& v, int32_t& id, uint32_t& flags) {
+    const uint4 hd = reinterpret_cast<const uint4*>(L)[0];
+    const double2* q = reinterpret_cast<const double2*>(L);
+    const double2 q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6], q7 = q[7];
+    id = (int32_t)hd.x;
+    flags = hd.y;
+    const d3 O = mk(q1.x, q1.y, q2.x), N = mk(q2.y, q3.x, q3.y);
+    const d3 U = mk(q4.x, q4.y, q5.x), V = mk(q5.y, q6.x, q6.y);
+    const double iu = q7.x, iv = q7.y;
     double denom = fdot(N, D);
     d3 w = sub(O, P);
     t = fdot(N, w) / denom;
@@ -210,10 +236,13 @@ __device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const
     d3 r = sub(X, O);
     u = fdot(r, U) * iu;
     v = fdot(r, V) * iv;
-    bool ok = denom != 0.0 && ((flags & FF_2SIDED) || !(denom > 0.0)) && t > 0.0 &&
-              !(u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0);
+    const uint32_t n = hd.w;
+    // non-short-circuit: every operand (the whole record) is needed here
+    bool ok = (denom != 0.0) & (((flags & FF_2SIDED) != 0u) | !(denom > 0.0)) & (t > 0.0) &
+              !((u < 0.0) | (u > 1.0) | (v < 0.0) | (v > 1.0)) & (id != skip) & (n >= 3u) & (hd.z != ~0u);
     if (!ok) return false;
-    return in_poly(n == 3 ? L->uv : poly + 2 * (size_t)L->poly_off, n, u, v);
+    if (n == 3) return in_tri(LU, u, v);
+    return in_poly(poly + 2 * (size_t)hd.z, n, u, v);
 }
 
 struct Ray32 { float ox, oy, oz, ix, iy, iz; };
@@ -231,19 +260,41 @@ __device__ __forceinline__ Ray32 make_ray32(d3 P, d3 D) {
 // Slab test of a node's 4 children against [0, tmax]: hit mask + entry t.
 // Boxes are padded at build time, so the FP32 test never drops a box that
 // holds an FP64 hit.
-__device__ __forceinline__ uint32_t node_hits(const BvhNode* __restrict__ nd, const Ray32& r, float tmax, float tn[4]) {
+// A node's 128 B as eight 16-byte loads issued together (one round trip).
+struct NodeRegs {
+    float4 lx, ly, lz, hx, hy, hz;
+    int4 ch;        // child words
+    uint4 meta;     // x: the four leaf counts (byte c = slot c), y: valid mask
+};
+static_assert(sizeof(NodeRegs) == sizeof(BvhNode), "NodeRegs mirrors BvhNode");
+
+__device__ __forceinline__ NodeRegs load_node(const BvhNode* __restrict__ nd) {
+    const float4* q = reinterpret_cast<const float4*>(nd);
+    NodeRegs n;
+    n.lx = q[0]; n.ly = q[1]; n.lz = q[2]; n.hx = q[3]; n.hy = q[4]; n.hz = q[5];
+    n.ch = reinterpret_cast<const int4*>(nd)[6];
+    n.meta = reinterpret_cast<const uint4*>(nd)[7];
+    return n;
+}
+
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const Ray32& r,
+                                     float tmax, float& tn) {
+    float tx0 = (lx - r.ox) * r.ix, tx1 = (hx - r.ox) * r.ix;
+    float ty0 = (ly - r.oy) * r.iy, ty1 = (hy - r.oy) * r.iy;
+    float tz0 = (lz - r.oz) * r.iz, tz1 = (hz - r.oz) * r.iz;
+    float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    tn = a;
+    return a <= b * 1.000001f;
+}
+
+__device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[4]) {
     uint32_t mask = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        float tx0 = (nd->lox[c] - r.ox) * r.ix, tx1 = (nd->hix[c] - r.ox) * r.ix;
-        float ty0 = (nd->loy[c] - r.oy) * r.iy, ty1 = (nd->hiy[c] - r.oy) * r.iy;
-        float tz0 = (nd->loz[c] - r.oz) * r.iz, tz1 = (nd->hiz[c] - r.oz) * r.iz;
-        float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        tn[c] = a;
-        if (a <= b * 1.000001f) mask |= 1u << c;
-    }
-    return mask & nd->valid;
+    if (slab(n.lx.x, n.ly.x, n.lz.x, n.hx.x, n.hy.x, n.hz.x, r, tmax, tn[0])) mask |= 1u;
+    if (slab(n.lx.y, n.ly.y, n.lz.y, n.hx.y, n.hy.y, n.hz.y, r, tmax, tn[1])) mask |= 2u;
+    if (slab(n.lx.z, n.ly.z, n.lz.z, n.hx.z, n.hy.z, n.hz.z, r, tmax, tn[2])) mask |= 4u;
+    if (slab(n.lx.w, n.ly.w, n.lz.w, n.hx.w, n.hy.w, n.hz.w, r, tmax, tn[3])) mask |= 8u;
+    return mask & n.meta.y;
 }
 
 __device__ __forceinline__ float widen(double t) {
@@ -273,12 +324,11 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     uint32_t sp = 0;
     int32_t node = root;
     for (;;) {
-        const BvhNode* nd = P.nodes + node;
+        const NodeRegs nd = load_node(P.nodes + node);
         float tn[4];
         uint32_t m = node_hits(nd, r, tbf, tn);
-        const int32_t ch0 = nd->child[0], ch1 = nd->child[1], ch2 = nd->child[2], ch3 = nd->child[3];
-        const uint32_t cnts = (uint32_t)nd->count[0] | ((uint32_t)nd->count[1] << 8) |
-                              ((uint32_t)nd->count[2] << 16) | ((uint32_t)nd->count[3] << 24);
+        const int32_t ch0 = nd.ch.x, ch1 = nd.ch.y, ch2 = nd.ch.z, ch3 = nd.ch.w;
+        const uint32_t cnts = nd.meta.x;
         // leaf children first (one loop for all slots, so lanes whose leaves
         // sit in different slots test their facets together)
         uint32_t lm = ((m & 1u) && ch0 < 0 ? 1u : 0u) | ((m & 2u) && ch1 < 0 ? 2u : 0u) |
@@ -289,15 +339,14 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
             int32_t ch = c == 0 ? ch0 : (c == 1 ? ch1 : (c == 2 ? ch2 : ch3));
             uint32_t first = (uint32_t)(~ch), cnt = (cnts >> (8u * c)) & 0xffu;
             for (uint32_t k = 0; k < cnt; ++k) {
-                const LeafFacet* L = P.leaves + first + k;
                 double t, u, v;
-                bool hit = leaf_test(L, P.poly, Pp, D, t, u, v);
-                int32_t f = L->id;
-                if (!hit || f == skip) continue;
+                int32_t f;
+                uint32_t fl;
+                bool hit = leaf_test(P.leaves + first + k, P.leaf_uv + first + k, P.poly, Pp, D, skip, t, u, v, f, fl);
+                if (!hit) continue;
                 if (!(t < h.t || (t == h.t && f < h.facet))) continue;
                 bool hard = true;
                 if constexpr (TRANSP) {
-                    uint32_t fl = L->flags;
                     if (!opaque_only && !(fl & FF_OPAQUE)) {
                         double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
                         if (op < 1.0) hard = (op > 0.0) && rng->coin(event, (uint32_t)(f - (int32_t)var->facet_base)) < op;
@@ -1102,25 +1151,25 @@ struct Tracer {
                 int32_t node = root;
                 double save_u = hu, save_v = hv;
                 for (;;) {
-                    const BvhNode* nd = P.nodes + node;
+                    const NodeRegs nd = load_node(P.nodes + node);
                     float tn[4];
                     uint32_t msk = node_hits(nd, r, tbf, tn);
-                    for (int c = 0; c < 4; ++c) {
-                        if (!(msk & (1u << c))) continue;
-                        int32_t ch = nd->child[c];
+                    while (msk) {
+                        uint32_t c = (uint32_t)__builtin_ctz(msk);
+                        msk &= msk - 1u;
+                        int32_t ch = c == 0 ? nd.ch.x : (c == 1 ? nd.ch.y : (c == 2 ? nd.ch.z : nd.ch.w));
                         if (ch >= 0) {
                             if (sp < P.stack_size) { stk[(size_t)sp * stride] = ch; ++sp; }
                             continue;
                         }
-                        uint32_t first = (uint32_t)(~ch), cnt = nd->count[c];
+                        uint32_t first = (uint32_t)(~ch), cnt = (nd.meta.x >> (8u * c)) & 0xffu;
                         for (uint32_t kk = 0; kk < cnt; ++kk) {
-                            const LeafFacet* L = P.leaves + first + kk;
-                            int32_t f = L->id;
-                            if (f == last_hit) continue;
-                            uint32_t fl = L->flags;
-                            if (fl & FF_OPAQUE) continue;
                             double t, u, v;
-                            if (!leaf_test(L, P.poly, pos, dir, t, u, v)) continue;
+                            int32_t f;
+                            uint32_t fl;
+                            bool hit = leaf_test(P.leaves + first + kk, P.leaf_uv + first + kk, P.poly, pos, dir, last_hit,
+                                                 t, u, v, f, fl);
+                            if (!hit || (fl & FF_OPAQUE)) continue;
                             if (!(t < travel)) continue;
                             double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
                             bool hard = op >= 1.0 || (op > 0.0 && rng.coin(nb_events, (uint32_t)local(f)) < op);
@@ -1379,7 +1428,7 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     TraceParams tp{};
-    tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaves = p.leaves;
+    tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaves = p.leaves; tp.leaf_uv = p.leaf_uv;
     d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
     int32_t skip = p.last_hit ? p.last_hit[i] : -1;
     if (skip >= 0) skip += (int32_t)p.facet_base;
