@@ -71,25 +71,41 @@ def algorithmic_bytes_per_hit(model, tex_hit_fraction, n_rays=20000):
     return b, nodes, prims
 
 
-def cpu_baseline(model, target_s=15.0):
+def cpu_baseline(model, run_s=5.0, repeats=3):
     """The CPU oracle (one tracer per OpenMP thread, thread-local tallies,
-    the reference's CPU parallelism) on a bounded sample of the workload."""
+    the reference's CPU parallelism) on a bounded sample of the workload:
+    a calibration run of at least a second sizes `repeats` runs of about
+    `run_s` seconds each (disjoint particle ranges); the median rate is
+    reported."""
     from oracle.mfcpu import CpuModel
     cm = CpuModel(model)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = int(os.environ.get("OMP_NUM_THREADS", threads))
-    n = 2000
-    t0 = time.perf_counter()
-    t = cm.run(n, seed=1, threads=threads)
-    dt = time.perf_counter() - t0
-    hits = int(t.glob[0]["nb_mc_hit"])
-    n2 = int(max(2000, min(2_000_000, n * target_s / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    t = cm.run(n2, seed=2, first=n, threads=threads)
-    dt = time.perf_counter() - t0
-    hits = int(t.glob[0]["nb_mc_hit"])
-    return {"value": hits / dt, "unit": "hits/s", "cores": threads, "kind": "port",
-            "sample": f"{n2} desorptions of the same beam-pipe model ({hits} hits, {dt:.1f} s), oracle/mfcpu.cpp"}
+    n, first = 2000, 0
+    while True:
+        t0 = time.perf_counter()
+        cm.run(n, seed=1, first=first, threads=threads)
+        dt = time.perf_counter() - t0
+        first += n
+        if dt >= 1.0 or n >= 4_000_000:
+            break
+        n *= 4
+    n_run = int(max(2000, min(20_000_000, n * run_s / max(dt, 1e-3))))
+    rates, total_hits, total_s = [], 0, 0.0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        t = cm.run(n_run, seed=2, first=first, threads=threads)
+        dt = time.perf_counter() - t0
+        first += n_run
+        hits = int(t.glob[0]["nb_mc_hit"])
+        rates.append(hits / dt)
+        total_hits += hits
+        total_s += dt
+    rates.sort()
+    return {"value": rates[len(rates) // 2], "unit": "hits/s", "cores": threads, "kind": "port",
+            "sample": f"median of {repeats} runs of {n_run} desorptions of the same beam-pipe model "
+                      f"({total_hits} hits in {total_s:.1f} s in all), oracle/mfcpu.cpp",
+            "runs": rates}
 
 
 def traffic_from_profile():
