@@ -133,11 +133,24 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
+    ap.add_argument("--workload", choices=("plain", "batched", "adjoint"), default="plain",
+                    help="plain: the beam-pipe job (default); batched: many uneven variants in one launch "
+                         "vs one launch per variant; adjoint: sticking sensitivities vs a forward run")
+    ap.add_argument("--variants", type=int, default=256)
+    ap.add_argument("--variant-des", type=str, default="2e4,2e5",
+                    help="desorptions per variant, log-uniform in [lo,hi]")
+    ap.add_argument("--adjoint-des", type=int, default=1_000_000)
+    ap.add_argument("--fd-des", type=int, default=100_000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: this build runs on a single GPU (see DESIGN.md for sharding)",
+        print("bench.py measures one MI355X: runs across several GPUs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        return batched_bench(args)
+    if args.workload == "adjoint":
+        return adjoint_bench(args)
 
     from molflow_sources_amd import geometry
     from molflow_sources_amd.simulation import GpuSimulation
@@ -223,6 +236,147 @@ def main():
     }
     if out["cpu_baseline"]:
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+def batched_bench(args):
+    """Uneven variants of the beam pipe (facet counts log-uniform in 2k-150k,
+    NEG sticking log-uniform in 1e-4..1e-1, seeded per variant): one
+    persistent launch over all of them vs one launch per variant, each
+    tracing its desorptions to the end. Setup (model upload, BVH build) is
+    outside both timed regions."""
+    import numpy as np
+
+    from molflow_sources_amd import geometry
+    from molflow_sources_amd.simulation import GpuSimulation
+    lo, hi = (float(x) for x in args.variant_des.split(","))
+    vs = geometry.variant_models(args.variants, seed=42)
+    rng = np.random.default_rng(7)
+    n_des = np.exp(rng.uniform(math.log(lo), math.log(hi), len(vs))).astype(np.uint64)
+    seeds = np.array([1_000_003 * (v + 1) + 42 for v in range(len(vs))], dtype=np.uint64)
+    max_events = 50_000_000
+    sim = GpuSimulation(0)
+    sim.load_variants(vs)
+    sim.rebuild_accel()
+    sim.reset()
+    tr = sim.autotune(int(n_des.sum() // 8))
+    sim.reset()
+    sim.run_variants(np.minimum(n_des, 1000), seeds, max_events=max_events)   # warm-up
+    sim.reset()
+    t0 = time.perf_counter()
+    r = sim.run_variants(n_des, seeds, max_events=max_events)
+    wall_b = time.perf_counter() - t0
+    hits_b, kms_b = r.hits, r.kernel_ms
+    batched_hits = [int(sim.read(variant=v).glob[0]["nb_mc_hit"]) for v in range(len(vs))]
+    sim.close()
+    p = GpuSimulation(0)
+    p.set_kernel_variant(tr.waves)
+    p.set_launch(tr.block, 0)
+    wall_p, hits_p, kms_p, same = 0.0, 0, 0.0, True
+    for v, m in enumerate(vs):
+        p.load(m)
+        p.rebuild_accel()
+        p.reset(seed=int(seeds[v]))
+        t0 = time.perf_counter()
+        r = p.run(int(n_des[v]), max_events=max_events)
+        wall_p += time.perf_counter() - t0
+        hits_p += r.hits
+        kms_p += r.kernel_ms
+        same &= r.hits == batched_hits[v]
+    p.close()
+    out = {
+        "metric": "aggregate MC hits/s, uneven beam-pipe variants: one batched launch vs one launch per variant",
+        "value": hits_b / wall_b, "unit": "hits/s", "n_gpus": 1, "higher_is_better": True,
+        "per_variant_launches_hits_per_s": hits_p / wall_p, "speedup_batched": (hits_b / wall_b) / (hits_p / wall_p),
+        "batched_wall_s": wall_b, "per_variant_wall_s": wall_p, "batched_kernel_ms": kms_b,
+        "per_variant_kernel_ms": kms_p, "hits": hits_b, "identical_hits_per_variant": bool(same),
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "beam_pipe_variants", "variants": len(vs),
+                   "facets_total": int(sum(m.n_facets for m in vs)),
+                   "facets_min": int(min(m.n_facets for m in vs)), "facets_max": int(max(m.n_facets for m in vs)),
+                   "desorptions_total": int(n_des.sum()), "desorptions_per_variant": [lo, hi],
+                   "launch": {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu}},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_bench(args):
+    """Sticking adjoint on the beam pipe in low-flux mode: 32 NEG sub-bands
+    (base sticking log-uniform in 1e-3..5e-2), 64 target wall facets. Cost
+    of a run with the path-score Jacobian vs the same forward run without
+    it, and every d(upstream . J0)/ds_g against central finite differences
+    (ds = 1e-3 s_g) with common random numbers, in particle batches."""
+    import numpy as np
+
+    from molflow_sources_amd import geometry
+    from molflow_sources_amd.simulation import GpuSimulation
+    m = geometry.beam_pipe_model()
+    m.settings.low_flux_mode = 1
+    m.settings.low_flux_cutoff = 1e-6
+    grp, base, tg = geometry.adjoint_setup(m, n_groups=32, n_targets=64, seed=42)
+    for g, s in enumerate(base):
+        m.facets["sticking"][grp == g] = s
+    up = np.random.default_rng(42).normal(size=len(tg))
+    max_events = 50_000_000
+    n = args.adjoint_des
+    # forward cost: same model, same sticking, no Jacobian
+    f = GpuSimulation(0)
+    f.load(m)
+    f.rebuild_accel()
+    f.reset(seed=42)
+    f.run(min(n, 20_000), max_events=max_events)
+    f.reset(seed=42)
+    rf = f.run(n, max_events=max_events)
+    f.close()
+    a = GpuSimulation(0)
+    a.load(m)
+    a.rebuild_accel()
+    a.adjoint_setup(grp, tg)
+    a.adjoint_set_sticking(base)
+    a.reset(seed=42)
+    a.run(min(n, 20_000), max_events=max_events)
+    a.reset(seed=42)
+    ra = a.run(n, max_events=max_events)
+    grad = a.adjoint_backward(up)
+    # finite differences with common random numbers, in batches for sigma
+    nb, per = 8, max(1, args.fd_des // 8)
+
+    def objective(s, b):
+        a.adjoint_set_sticking(s)
+        a.reset(seed=42, particle_offset=b * per)
+        a.run(per, max_events=max_events)
+        j0, jac = a.adjoint_read()
+        return float(up @ j0) / per, (up @ jac) / per
+
+    adj_b = [objective(base, b)[1] for b in range(nb)]
+    z = []
+    for g in range(len(base)):
+        diffs = []
+        for b in range(nb):
+            ds = 1e-3 * base[g]
+            sp, sm = base.copy(), base.copy()
+            sp[g] += ds
+            sm[g] -= ds
+            fp, _ = objective(sp, b)
+            fm, _ = objective(sm, b)
+            diffs.append((fp - fm) / (2 * ds) - adj_b[b][g])
+        d = np.array(diffs)
+        sem = d.std(ddof=1) / math.sqrt(nb)
+        z.append(float(d.mean() / sem) if sem > 0 else 0.0)
+    a.close()
+    out = {
+        "metric": "adjoint cost vs one forward run, beam pipe, 32 sticking groups x 64 targets",
+        "value": ra.kernel_ms / rf.kernel_ms, "unit": "x forward kernel time", "n_gpus": 1,
+        "higher_is_better": False, "forward_kernel_ms": rf.kernel_ms, "adjoint_kernel_ms": ra.kernel_ms,
+        "forward_hits_per_s": rf.hits / (rf.kernel_ms / 1e3), "adjoint_hits_per_s": ra.hits / (ra.kernel_ms / 1e3),
+        "hits": ra.hits, "gradient": [float(x) for x in grad],
+        "fd_check": {"groups_within_2sigma": int(sum(abs(x) <= 2 for x in z)), "groups": len(z),
+                     "max_abs_z": float(max(abs(x) for x in z)), "z": z,
+                     "batches": nb, "desorptions_per_batch": per},
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "beam_pipe_low_flux_adjoint", "facets": m.n_facets, "desorptions": n,
+                   "groups": len(base), "targets": len(tg), "low_flux_cutoff": 1e-6},
+    }
     print(json.dumps(out), flush=True)
 
 

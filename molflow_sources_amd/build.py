@@ -36,5 +36,23 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "run_model.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "run_model")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ host driver over include/molflow_gpu.hpp (used by the GPU
+    tests); links the in-tree library through an $ORIGIN-relative rpath."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L", HERE, "-lmolflow_gpu", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-Wl,-rpath,$ORIGIN/../../molflow_sources_amd", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_driver(verbose=True))

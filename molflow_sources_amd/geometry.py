@@ -232,11 +232,16 @@ def _orient_ports(m: Model, kinds: np.ndarray, ports, radius: float) -> None:
     pumps = np.nonzero(kinds == 3)[0]
     for p, (zc, phic, _) in enumerate(ports):
         ed = np.array([math.cos(phic), math.sin(phic), 0.0])
-        for f in port_tris[p * per_port:(p + 1) * per_port]:
-            P = m.vertices[m.polygons[f]]
-            c = P.mean(0)
-            axis_pt = np.array([0.0, 0.0, zc]) + np.dot(c, ed) * ed
-            if np.dot(_newell(P), axis_pt - c) < 0:
+        ids = port_tris[p * per_port:(p + 1) * per_port]
+        if len(ids):
+            P = m.vertices[np.asarray([m.polygons[f] for f in ids], dtype=np.int64)]   # (n, 3, 3)
+            c = P.mean(1)
+            axis_pt = np.array([0.0, 0.0, zc]) + (c @ ed)[:, None] * ed
+            Q = np.roll(P, -1, axis=1)
+            nw = np.stack([np.sum((P[..., 1] - Q[..., 1]) * (P[..., 2] + Q[..., 2]), axis=1),
+                           np.sum((P[..., 2] - Q[..., 2]) * (P[..., 0] + Q[..., 0]), axis=1),
+                           np.sum((P[..., 0] - Q[..., 0]) * (P[..., 1] + Q[..., 1]), axis=1)], axis=1)
+            for f in ids[np.einsum("ij,ij->i", nw, axis_pt - c) < 0]:
                 m.polygons[f] = m.polygons[f][::-1]
         P = m.vertices[m.polygons[pumps[p]]]
         if np.dot(_newell(P), -ed) < 0:

@@ -113,14 +113,18 @@ class Model:
     def facet_areas(self) -> np.ndarray:
         """Polygon areas (cm^2) from the 3D vertices (Newell's formula)."""
         out = np.empty(self.n_facets)
+        by_len: dict[int, list[int]] = {}
         for i, poly in enumerate(self.polygons):
-            P = self.vertices[np.asarray(poly)]
-            n = np.zeros(3)
-            Q = np.roll(P, -1, axis=0)
-            n[0] = np.sum((P[:, 1] - Q[:, 1]) * (P[:, 2] + Q[:, 2]))
-            n[1] = np.sum((P[:, 2] - Q[:, 2]) * (P[:, 0] + Q[:, 0]))
-            n[2] = np.sum((P[:, 0] - Q[:, 0]) * (P[:, 1] + Q[:, 1]))
-            out[i] = 0.5 * np.linalg.norm(n)
+            by_len.setdefault(len(poly), []).append(i)
+        for k, ids in by_len.items():
+            # all polygons with k vertices at once: P[f, j] and its successor
+            idx = np.asarray([self.polygons[i] for i in ids], dtype=np.int64)
+            P = self.vertices[idx]
+            Q = np.roll(P, -1, axis=1)
+            n = np.stack([np.sum((P[..., 1] - Q[..., 1]) * (P[..., 2] + Q[..., 2]), axis=1),
+                          np.sum((P[..., 2] - Q[..., 2]) * (P[..., 0] + Q[..., 0]), axis=1),
+                          np.sum((P[..., 0] - Q[..., 0]) * (P[..., 1] + Q[..., 1]), axis=1)], axis=1)
+            out[np.asarray(ids)] = 0.5 * np.linalg.norm(n, axis=1)
         return out
 
     def view(self) -> ModelView:

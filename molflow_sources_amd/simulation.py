@@ -57,6 +57,7 @@ class GpuSimulation:
         self.model = None
         self.models = None
         self._views = []
+        self.generation = 0      # counts runs; the adjoint binding checks it
 
     def close(self):
         if getattr(self, "h", None):
@@ -89,14 +90,17 @@ class GpuSimulation:
         self._check(self._lib.mfg_build_accel(self.h, width, leaf_size), "mfg_build_accel")
 
     def reset(self, seed: int = 42, particle_offset: int = 0) -> None:
+        self.generation += 1
         self._check(self._lib.mfg_reset(self.h, seed, particle_offset), "mfg_reset")
 
     def run(self, n_des: int, max_events: int = 0) -> A.StepResult:
+        self.generation += 1
         r = A.StepResult()
         self._check(self._lib.mfg_run(self.h, int(n_des), int(max_events), C.byref(r)), "mfg_run")
         return r
 
     def run_variants(self, n_des, seeds, max_events: int = 0) -> A.StepResult:
+        self.generation += 1
         n = np.ascontiguousarray(n_des, dtype=np.uint64)
         s = np.ascontiguousarray(seeds, dtype=np.uint64)
         r = A.StepResult()

@@ -45,3 +45,43 @@ def z_two_proportions(k1, n1, k2, n2):
 
 def z_means(m1, s1, m2, s2):
     return (m1 - m2) / math.sqrt(max(s1 ** 2 + s2 ** 2, 1e-300))
+
+
+def write_model_file(model, path):
+    """Model file read by tests/cpp/run_model (single structure, no
+    parameters or moments): header, vertices, polygon offsets and indices,
+    then the raw facet and settings structs of molflow_gpu.h."""
+    import ctypes as C
+    import struct
+
+    import numpy as np
+    v = model.view()
+    nv, nf = v.n_vertices, v.n_facets
+    off = np.ctypeslib.as_array(v.facet_vertex_offset, shape=(nf + 1,)).astype(np.uint32)
+    idx = np.ctypeslib.as_array(v.facet_vertex_index, shape=(int(off[-1]),)).astype(np.uint32)
+    verts = np.ctypeslib.as_array(v.vertices, shape=(3 * nv,)).astype(np.float64)
+    from molflow_sources_amd._abi import FacetDesc, Settings
+    fbytes = C.string_at(v.facets, nf * C.sizeof(FacetDesc))
+    sbytes = C.string_at(v.settings, C.sizeof(Settings))
+    with open(path, "wb") as fh:
+        fh.write(b"MFGM1\0\0\0")
+        fh.write(struct.pack("<4I", nv, nf, len(idx), 0))
+        fh.write(verts.tobytes())
+        fh.write(off.tobytes())
+        fh.write(idx.tobytes())
+        fh.write(fbytes)
+        fh.write(sbytes)
+
+
+def read_driver_output(path):
+    import numpy as np
+    head, rows = {}, []
+    for line in open(path):
+        parts = line.split()
+        if parts[0].isdigit():
+            rows.append((int(parts[1]), float(parts[2])))
+        else:
+            head[parts[0]] = float(parts[1]) if "." in parts[1] or "e" in parts[1] else int(parts[1])
+    hits = np.array([r[0] for r in rows], dtype=np.int64)
+    absq = np.array([r[1] for r in rows])
+    return head, hits, absq
