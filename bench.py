@@ -139,8 +139,8 @@ def main():
     ap.add_argument("--variants", type=int, default=256)
     ap.add_argument("--variant-des", type=str, default="2e4,2e5",
                     help="desorptions per variant, log-uniform in [lo,hi]")
-    ap.add_argument("--adjoint-des", type=int, default=1_000_000)
-    ap.add_argument("--fd-des", type=int, default=100_000)
+    ap.add_argument("--adjoint-des", type=int, default=200_000)
+    ap.add_argument("--fd-des", type=int, default=40_000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: runs across several GPUs are not part of this build",
@@ -239,6 +239,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def progress(msg):
+    """One line per finished stage on stderr (long workloads stay visibly alive)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def batched_bench(args):
     """Uneven variants of the beam pipe (facet counts log-uniform in 2k-150k,
     NEG sticking log-uniform in 1e-4..1e-1, seeded per variant): one
@@ -251,6 +256,7 @@ def batched_bench(args):
     from molflow_sources_amd.simulation import GpuSimulation
     lo, hi = (float(x) for x in args.variant_des.split(","))
     vs = geometry.variant_models(args.variants, seed=42)
+    progress(f"{len(vs)} variants, {sum(m.n_facets for m in vs)} facets")
     rng = np.random.default_rng(7)
     n_des = np.exp(rng.uniform(math.log(lo), math.log(hi), len(vs))).astype(np.uint64)
     seeds = np.array([1_000_003 * (v + 1) + 42 for v in range(len(vs))], dtype=np.uint64)
@@ -259,7 +265,9 @@ def batched_bench(args):
     sim.load_variants(vs)
     sim.rebuild_accel()
     sim.reset()
+    progress("batched accel built")
     tr = sim.autotune(int(n_des.sum() // 8))
+    progress(f"autotune: {tr.waves} waves/SIMD, block {tr.block}")
     sim.reset()
     sim.run_variants(np.minimum(n_des, 1000), seeds, max_events=max_events)   # warm-up
     sim.reset()
@@ -268,6 +276,7 @@ def batched_bench(args):
     wall_b = time.perf_counter() - t0
     hits_b, kms_b = r.hits, r.kernel_ms
     batched_hits = [int(sim.read(variant=v).glob[0]["nb_mc_hit"]) for v in range(len(vs))]
+    progress(f"batched launch: {hits_b} hits in {wall_b:.2f} s")
     sim.close()
     p = GpuSimulation(0)
     p.set_kernel_variant(tr.waves)
@@ -283,6 +292,8 @@ def batched_bench(args):
         hits_p += r.hits
         kms_p += r.kernel_ms
         same &= r.hits == batched_hits[v]
+        if v % 32 == 31:
+            progress(f"per-variant launches: {v + 1} done")
     p.close()
     out = {
         "metric": "aggregate MC hits/s, uneven beam-pipe variants: one batched launch vs one launch per variant",
@@ -328,6 +339,7 @@ def adjoint_bench(args):
     f.reset(seed=42)
     rf = f.run(n, max_events=max_events)
     f.close()
+    progress(f"forward: {rf.hits} hits in {rf.kernel_ms:.1f} ms")
     a = GpuSimulation(0)
     a.load(m)
     a.rebuild_accel()
@@ -338,6 +350,7 @@ def adjoint_bench(args):
     a.reset(seed=42)
     ra = a.run(n, max_events=max_events)
     grad = a.adjoint_backward(up)
+    progress(f"adjoint: {ra.hits} hits in {ra.kernel_ms:.1f} ms")
     # finite differences with common random numbers, in batches for sigma
     nb, per = 8, max(1, args.fd_des // 8)
 
@@ -360,6 +373,7 @@ def adjoint_bench(args):
             fp, _ = objective(sp, b)
             fm, _ = objective(sm, b)
             diffs.append((fp - fm) / (2 * ds) - adj_b[b][g])
+        progress(f"finite differences: group {g + 1}/{len(base)}")
         d = np.array(diffs)
         sem = d.std(ddof=1) / math.sqrt(nb)
         z.append(float(d.mean() / sem) if sem > 0 else 0.0)
