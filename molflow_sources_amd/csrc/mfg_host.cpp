@@ -1009,8 +1009,8 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
 
 int default_waves(uint32_t feats) {
     uint32_t k = mfg_trace_kernel_features(feats);
-    if (k & FEAT_ADJOINT) return 1;
-    return (k & FEAT_ALL) ? 2 : 3;
+    if ((k & FEAT_ALL & ~FEAT_LOWFLUX) == 0) return 3;   // lean and low-flux sets
+    return (k & FEAT_ADJOINT) ? 1 : 2;
 }
 
 void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& waves) {

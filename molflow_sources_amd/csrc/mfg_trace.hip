@@ -1454,16 +1454,21 @@ __global__ void k_backward(const double* jac, const double* up, double* grad, ui
     grad[g] = s * scale;
 }
 
+// Instantiated feature sets: a model runs on the smallest one covering its
+// features (fewer live registers, higher occupancy).
 constexpr uint32_t kLean = 0;
+constexpr uint32_t kLowFlux = FEAT_LOWFLUX;
+constexpr uint32_t kLowFluxAdj = FEAT_LOWFLUX | FEAT_ADJOINT;
 constexpr uint32_t kFull = FEAT_ALL;
 constexpr uint32_t kFullAdj = FEAT_ALL | FEAT_ADJOINT;
 
 }  // namespace
 
 uint32_t mfg_trace_kernel_features(uint32_t model_features) {
-    if (model_features & FEAT_ADJOINT) return kFullAdj;
-    if (model_features & FEAT_ALL) return kFull;
-    return kLean;
+    const uint32_t rest = model_features & FEAT_ALL & ~FEAT_LOWFLUX;
+    if (model_features & FEAT_ADJOINT) return rest ? kFullAdj : kLowFluxAdj;
+    if (rest) return kFull;
+    return (model_features & FEAT_LOWFLUX) ? kLowFlux : kLean;
 }
 
 namespace {
@@ -1473,7 +1478,13 @@ TraceKernel kernel_for(uint32_t features, int waves) {
     static TraceKernel lean[4] = {nullptr, k_trace<kLean, 2>, k_trace<kLean, 3>, k_trace<kLean, 4>};
     static TraceKernel full[4] = {k_trace<kFull, 1>, k_trace<kFull, 2>, k_trace<kFull, 3>, nullptr};
     static TraceKernel adj[4] = {k_trace<kFullAdj, 1>, k_trace<kFullAdj, 2>, nullptr, nullptr};
-    TraceKernel* t = features == kLean ? lean : (features == kFull ? full : adj);
+    static TraceKernel lowflux[4] = {nullptr, k_trace<kLowFlux, 2>, k_trace<kLowFlux, 3>, k_trace<kLowFlux, 4>};
+    static TraceKernel lowflux_adj[4] = {nullptr, k_trace<kLowFluxAdj, 2>, k_trace<kLowFluxAdj, 3>,
+                                         k_trace<kLowFluxAdj, 4>};
+    TraceKernel* t = features == kLean ? lean
+                   : features == kFull ? full
+                   : features == kLowFlux ? lowflux
+                   : features == kLowFluxAdj ? lowflux_adj : adj;
     if (waves < 1) waves = 1;
     if (waves > 4) waves = 4;
     for (int w = waves; w >= 1; --w) if (t[w - 1]) return t[w - 1];
@@ -1484,7 +1495,9 @@ TraceKernel kernel_for(uint32_t features, int waves) {
 
 int mfg_trace_variants(uint32_t features, int* waves_out, int cap) {
     uint32_t f = mfg_trace_kernel_features(features);
-    int lo = f == kLean ? 2 : 1, hi = f == kLean ? 4 : (f == kFull ? 3 : 2);
+    int lo = 1, hi = 2;
+    if (f == kLean || f == kLowFlux || f == kLowFluxAdj) { lo = 2; hi = 4; }
+    else if (f == kFull) hi = 3;
     int n = 0;
     for (int w = lo; w <= hi && n < cap; ++w) waves_out[n++] = w;
     return n;
