@@ -151,7 +151,10 @@ typedef struct mfg_model_view {
     const uint64_t* og_map_offset;        /* n_facets+1 into og_map_values, nullable */
     const double* og_map_values;          /* Pa*m3/s per cell */
     const uint64_t* am_pdf_offset;        /* n_facets+1 into am_pdf (DES_ANGLEMAP), nullable */
-    const uint64_t* am_pdf;               /* recorded incident-angle counts */
+    const uint64_t* am_pdf;               /* recorded incident-angle counts of each DES_ANGLEMAP
+                                             facet, (am_theta_lower_res + am_theta_higher_res)
+                                             lines of am_phi_width counts (LoaderXML.cpp:1043-1073;
+                                             sampled as in AnglemapGeneration.cpp:15-410) */
 } mfg_model_view;
 
 /* ---- tallies (the output contract, GlobalSimuState) -------------------- */

@@ -359,6 +359,7 @@ struct HostModel {
     std::vector<double> src_cum;
     std::vector<int32_t> src_facet;   // local
     std::vector<double> og_cdf;
+    std::vector<double> am_src;       // DES_ANGLEMAP blocks (variant-local offsets in DPhys)
     double latest = 0, total_des = 0, final_rate = 0, c_factor = 1.0;
     uint32_t n_slots = 1;
     uint64_t n_tex = 0, n_dir = 0, n_prof = 0, n_am = 0, n_fh = 0;
@@ -367,6 +368,54 @@ struct HostModel {
     std::vector<std::vector<int>> struct_facets;
     double extent = 1.0;
 };
+
+// DES_ANGLEMAP source table of one facet (InitializeAngleMap,
+// src/Simulation/MolflowSimFacet.cpp:92-222): phi line sums, theta CDF at
+// line midpoints, per-line midpoint phi CDFs and normalised phi pdfs; the
+// lower and higher theta parts are stored as one run of L + H lines.
+void angle_source_block(std::vector<double>& out, const mfg_facet_desc& d, const uint64_t* pdf, uint64_t n,
+                        uint32_t f) {
+    const int W = (int)d.am_phi_width, L = (int)d.am_theta_lower_res, H = (int)d.am_theta_higher_res, T = L + H;
+    if (W <= 0 || T <= 0 || n != (uint64_t)T * (uint64_t)W)
+        throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": DES_ANGLEMAP map does not match its shape");
+    std::vector<uint64_t> sum(T, 0);
+    uint64_t lower = 0, total;
+    for (int t = 0; t < L; ++t) {
+        for (int p = 0; p < W; ++p) sum[t] += pdf[(size_t)t * W + p];
+        lower += sum[t];
+    }
+    total = lower;
+    for (int t = L; t < T; ++t) {
+        for (int p = 0; p < W; ++p) sum[t] += pdf[(size_t)t * W + p];
+        total += sum[t];
+    }
+    if (total == 0) throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": DES_ANGLEMAP map is all zero");
+    const double ratio = (double)lower / (double)total;
+    const size_t base = out.size();
+    out.resize(base + 6 + 2 * (size_t)T + 2 * (size_t)T * W);
+    double* b = out.data() + base;
+    b[0] = W; b[1] = L; b[2] = H; b[3] = d.am_theta_limit; b[4] = ratio; b[5] = (double)total;
+    double *line = b + 6, *cdf = line + T, *pc = cdf + T, *pp = pc + (size_t)T * W;
+    const double tn = 1.0 / (double)total;
+    for (int t = 0; t < T; ++t) {
+        line[t] = (double)sum[t];
+        if (t == 0) cdf[t] = 0.5 * (double)sum[0] * tn;
+        else if (t == L) cdf[t] = ratio + 0.5 * (double)sum[L] * tn;
+        else cdf[t] = cdf[t - 1] + (double)(sum[t - 1] + sum[t]) * 0.5 * tn;
+        const double pn = 1.0 / (double)sum[t];
+        for (int p = 0; p < W; ++p) {
+            const size_t i = (size_t)t * W + p;
+            if (sum[t] == 0) {
+                pc[i] = (0.5 + (double)p) / (double)W;
+                pp[i] = 1.0 / (double)W;
+            } else {
+                if (p == 0) pc[i] = 0.5 * (double)pdf[(size_t)t * W] * pn;
+                else pc[i] = pc[i - 1] + (double)(pdf[i - 1] + pdf[i]) * 0.5 * pn;
+                pp[i] = (double)pdf[i] * pn;
+            }
+        }
+    }
+}
 
 void prepare(HostModel& M, const mfg_model_view* v) {
     if (!v) throw Err(MFG_ERR_INVALID, "null model view");
@@ -509,7 +558,14 @@ void prepare(HostModel& M, const mfg_model_view* v) {
         if (d.count_refl) fl |= FF_COUNT_REFL;
         if (d.count_trans) fl |= FF_COUNT_TRANS;
         if (d.count_dir) fl |= FF_COUNT_DIR;
-        if (d.desorb_type == MFG_DES_ANGLEMAP) throw Err(MFG_ERR_INVALID, "DES_ANGLEMAP desorption is not supported");
+        ph.am_src_off = -1;
+        if (d.desorb_type == MFG_DES_ANGLEMAP) {
+            if (!v->am_pdf_offset || !v->am_pdf || v->am_pdf_offset[f + 1] == v->am_pdf_offset[f])
+                throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": DES_ANGLEMAP without a recorded angle map");
+            ph.am_src_off = (int64_t)M.am_src.size();
+            angle_source_block(M.am_src, d, v->am_pdf + v->am_pdf_offset[f], v->am_pdf_offset[f + 1] - v->am_pdf_offset[f], f);
+            feat |= FEAT_AMSRC;
+        }
         // textures
         if (d.is_textured && d.tex_width_precise > 0 && d.tex_height_precise > 0) {
             fl |= FF_TEXTURE;
@@ -672,7 +728,7 @@ struct mfg_context {
     // device data
     DevBuf<DFacet> facets;
     DevBuf<DPhys> phys;
-    DevBuf<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_start, mom_end, tex_inc;
+    DevBuf<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_start, mom_end, tex_inc, am_src;
     DevBuf<int32_t> src_facet, roots;
     DevBuf<LeafFacet> leaves;
     DevBuf<LeafUV> leaf_uv;
@@ -750,7 +806,7 @@ void upload_models(mfg_context* c) {
     reset_graph(c);
     std::vector<DFacet> facets;
     std::vector<DPhys> phys;
-    std::vector<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_s, mom_e, tex_inc;
+    std::vector<double> poly, src_cum, og_cdf, tab_x, tab_y, mom_s, mom_e, tex_inc, am_src;
     std::vector<int32_t> src_facet;
     std::vector<uint32_t> guides;
     std::vector<DTable> params, ids;
@@ -772,10 +828,12 @@ void upload_models(mfg_context* c) {
             facets.push_back(F);
             DPhys ph = M.phys[f];
             if (ph.og_off >= 0) ph.og_off += (int64_t)og_cdf.size();
+            if (ph.am_src_off >= 0) ph.am_src_off += (int64_t)am_src.size();
             phys.push_back(ph);
         }
         poly.insert(poly.end(), M.poly.begin(), M.poly.end());
         og_cdf.insert(og_cdf.end(), M.og_cdf.begin(), M.og_cdf.end());
+        am_src.insert(am_src.end(), M.am_src.begin(), M.am_src.end());
         dv.src_base = (uint32_t)src_cum.size();
         dv.src_n = (uint32_t)M.src_cum.size();
         src_cum.insert(src_cum.end(), M.src_cum.begin(), M.src_cum.end());
@@ -852,6 +910,7 @@ void upload_models(mfg_context* c) {
     c->h_facets = facets;
     c->h_poly = poly;
     c->src_cum.upload(src_cum, s); c->src_facet.upload(src_facet, s); c->og_cdf.upload(og_cdf, s);
+    c->am_src.upload(am_src, s);
     c->tab_x.upload(tab_x, s); c->tab_y.upload(tab_y, s); c->guides.upload(guides, s); c->params.upload(params, s); c->ids.upload(ids, s);
     c->mom_start.upload(mom_s, s); c->mom_end.upload(mom_e, s); c->tex_inc.upload(tex_inc, s);
     c->fc.alloc(std::max<uint64_t>(fc, 1)); c->force.alloc(std::max<uint64_t>(force, 1));
@@ -988,7 +1047,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
     p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size(); p.stack_size = c->stack_size;
-    p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p;
+    p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p; p.am_src = c->am_src.p;
     p.guides = c->guides.p; p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
     p.mom_start = c->mom_start.p; p.mom_end = c->mom_end.p;
     p.fc = c->fc.p; p.force = c->force.p; p.tex = c->tex.p; p.tex_inc = c->tex_inc.p; p.dir = c->dir.p;

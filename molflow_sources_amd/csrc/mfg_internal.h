@@ -37,8 +37,9 @@ enum : uint32_t {
     FEAT_LOG = 1u << 10,
     FEAT_OGMAP = 1u << 11,
     FEAT_TRACE = 1u << 12,
-    FEAT_ADJOINT = 1u << 13,
-    FEAT_ALL = (1u << 13) - 1,
+    FEAT_AMSRC = 1u << 13,        // DES_ANGLEMAP sources
+    FEAT_ADJOINT = 1u << 14,
+    FEAT_ALL = (1u << 14) - 1,
 };
 
 // facet flags (DFacet::flags, DPhys::flags)
@@ -107,6 +108,8 @@ struct alignas(16) DPhys {
     int64_t og_off;                 // into og cdf array (global), -1
     int32_t desorb_type;
     int32_t pad_;
+    int64_t am_src_off;             // DES_ANGLEMAP source table in the pool (global), -1
+    int64_t pad2_;
 };
 
 struct alignas(16) BvhNode {
@@ -230,6 +233,10 @@ struct TraceParams {
     const DTable* ids;
     const double* mom_start;
     const double* mom_end;
+    // DES_ANGLEMAP source tables, one block per facet:
+    // [W, L, H, theta limit, lower ratio, total count, line sums (T),
+    //  theta CDF (T), phi CDFs (T*W), phi pdfs (T*W)], T = L + H
+    const double* am_src;
     // tallies
     mfg_facet_counters* fc;
     double* force;

@@ -304,6 +304,155 @@ __device__ __forceinline__ float widen(double t) {
 
 struct HitRec { int32_t facet; double t, u, v; };
 
+// ---------------------------------------------------------------- DES_ANGLEMAP
+// Sampling of a recorded incident-angle map (AnglemapGeneration.cpp:15-410),
+// on the table block described in TraceParams::am_src. lower_index is the
+// index of the last CDF value <= key (-1 below the first); weigh(a, b, w) =
+// a + (b - a) * w.
+struct AmSrc {
+    int W, L, H;
+    double limit, ratio, total;
+    const double *line, *cdf, *pc, *pp;
+};
+__device__ __forceinline__ AmSrc am_view(const double* b) {
+    AmSrc A;
+    A.W = (int)b[0]; A.L = (int)b[1]; A.H = (int)b[2];
+    A.limit = b[3]; A.ratio = b[4]; A.total = b[5];
+    const int T = A.L + A.H;
+    A.line = b + 6; A.cdf = A.line + T; A.pc = A.cdf + T; A.pp = A.pc + (size_t)T * A.W;
+    return A;
+}
+__device__ __forceinline__ double am_weigh(double a, double b, double w) { return a + (b - a) * w; }
+__device__ int am_lower_index(double key, const double* A, int n) {
+    int lo = 0, hi = n;   // first element > key
+    while (lo < hi) { int mid = (lo + hi) / 2; if (A[mid] <= key) lo = mid + 1; else hi = mid; }
+    return lo - 1;
+}
+__device__ int am_weighed_lower_index(double key, double w, const double* A, const double* B, int n) {
+    int lo = 0, hi = n;
+    while (lo < hi) { int mid = (lo + hi) / 2; if (am_weigh(A[mid], B[mid], w) <= key) lo = mid + 1; else hi = mid; }
+    return lo - 1;
+}
+__device__ __forceinline__ double am_theta(const AmSrc& A, double ti) {
+    if ((size_t)ti < (size_t)A.L) return A.limit * ti / (double)A.L;
+    return A.limit + (kPi / 2.0 - A.limit) * (ti - (double)A.L) / (double)A.H;
+}
+__device__ __forceinline__ double am_phi(const AmSrc& A, double pi_) {
+    double w = (double)A.W;
+    double c = (pi_ < w) ? pi_ : pi_ - w;
+    return -kPi + 2.0 * kPi * c / w;
+}
+__device__ double am_phi_pdf(const AmSrc& A, double ti, int plo) {
+    const int W = A.W, T = A.L + A.H;
+    const size_t p = (size_t)(((plo % W) + W) % W);
+    if (ti < 0.5) return A.pp[p];
+    if (ti > (double)T - 0.5) return A.pp[(size_t)(T - 1) * W + p];
+    size_t tl = (size_t)(ti - 0.5);
+    double ov = ti - 0.5 - (double)tl;
+    return am_weigh(A.pp[tl * W + p], A.pp[(tl + 1) * W + p], ov);
+}
+__device__ double am_phi_cdf(const AmSrc& A, double ti, int plo) {
+    const int W = A.W, L = A.L, T = A.L + A.H;
+    auto line = [&](size_t t) { return (plo < W) ? A.pc[t * W + plo] : 1.0 + A.pc[t * W]; };
+    if (ti < (double)L) {
+        if (ti < 0.5) return line(0);
+        if (ti > (double)L - 0.5) return line((size_t)(L - 1));
+    } else {
+        if (ti < (double)L + 0.5) return line((size_t)L);
+        if (ti > (double)T - 0.5) return line((size_t)(T - 1));
+    }
+    size_t tl = (size_t)(ti - 0.5);
+    double ov = ti - 0.5 - (double)tl;
+    return am_weigh(line(tl), line(tl + 1), ov);
+}
+__device__ void am_generate_theta(const AmSrc& A, double r, double& theta, int& tlo, double& ov) {
+    const int L = A.L, H = A.H;
+    const double* cdf = A.cdf;
+    auto quad = [&](int t, double c) {
+        double step = am_theta(A, (double)t + 1.5) - am_theta(A, (double)t + 0.5);
+        double b = A.line[t] / A.total / step;
+        double a = 0.5 * (A.line[t + 1] - A.line[t]) / A.total / (step * step);
+        double dy = r - c;
+        double dx = (-b + sqrt(b * b + 4 * a * dy)) / (2 * a);
+        return dx / step;
+    };
+    if (r < A.ratio) {
+        tlo = am_lower_index(r, cdf, L);
+        if (tlo == -1) {
+            ov = 0.5 * r / cdf[0];
+            theta = am_theta(A, ov);
+        } else if (tlo == L - 1) {
+            ov = 0.5 * (r - cdf[tlo]) / (A.ratio - cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        } else {
+            if (A.line[tlo] == A.line[tlo + 1]) ov = (r - cdf[tlo]) / (cdf[tlo + 1] - cdf[tlo]);
+            else ov = quad(tlo, cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        }
+    } else {
+        tlo = L + am_lower_index(r, cdf + L, H);
+        if (tlo == L - 1) {
+            ov = 0.5 * (r - A.ratio) / (cdf[L] - A.ratio);
+            theta = am_theta(A, (double)L + ov);
+        } else if (tlo == L + H - 1) {
+            ov = 0.5 * (r - cdf[tlo]) / (1.0 - cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        } else {
+            if (A.line[tlo] == A.line[tlo + 1]) ov = (r - cdf[tlo]) / (cdf[tlo + 1] - cdf[tlo]);
+            else ov = quad(tlo, cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        }
+    }
+}
+__device__ double am_generate_phi(const AmSrc& A, int tlo, double ov, double r) {
+    const int W = A.W, L = A.L, T = A.L + A.H;
+    if (W == 1) return -kPi + 2.0 * kPi * r;
+    int plo;
+    double w;
+    const double* C = A.pc;
+    auto edge = [&](int t) {
+        r += C[(size_t)t * W];
+        plo = am_lower_index(r, C + (size_t)t * W, W);
+        w = ov;
+    };
+    auto between = [&](int t) {
+        double div = A.line[t] * (1.0 - ov) + A.line[t + 1] * ov;
+        w = div > 0.0 ? (ov * A.line[t + 1]) / div : ov;
+        r += am_weigh(C[(size_t)t * W], C[(size_t)(t + 1) * W], w);
+        plo = am_weighed_lower_index(r, w, C + (size_t)t * W, C + (size_t)(t + 1) * W, W);
+    };
+    if (tlo < L) {
+        if (tlo == -1) edge(0);
+        else if (tlo == L - 1) edge(tlo);
+        else between(tlo);
+    } else {
+        if (tlo == L - 1) edge(L);
+        else if (tlo == T - 1) edge(tlo);
+        else between(tlo);
+    }
+    const double ti = (double)tlo + 0.5 + w;
+    double pov;
+    if (plo == -1) {
+        pov = 0.5 * r / am_phi_cdf(A, ti, 0);
+        return am_phi(A, pov);
+    }
+    double p0 = am_phi_pdf(A, ti, plo), p1 = am_phi_pdf(A, ti, plo + 1);
+    if (p0 == p1) {
+        double c0 = am_phi_cdf(A, ti, plo);
+        pov = (r - c0) / (am_phi_cdf(A, ti, plo + 1) - c0);
+    } else {
+        double step = 2.0 * kPi / (double)W;
+        double c = am_phi_cdf(A, ti, plo);
+        double b = p0 / step;
+        double a = 0.5 * (p1 - p0) / (step * step);
+        double dy = r - c;
+        double D = b * b + 4 * a * dy;
+        if (D < 0) pov = (r - c) / (am_phi_cdf(A, ti, ((plo + 1) % W + W) % W) - c);
+        else pov = ((-b + sqrt(b * b + 4 * a * dy)) / (2 * a)) / step;
+    }
+    return am_phi(A, (double)plo + 0.5 + pov);
+}
+
 __device__ __forceinline__ double td_opacity(const TraceParams& P, const DVariant* var, int32_t f, double time) {
     const DPhys& ph = P.phys[f];
     const DTable tb = P.params[var->param_base + ph.opacity_param];
@@ -751,6 +900,17 @@ struct Tracer {
         if (type == MFG_DES_COSINE_N) {
             double c = pow(rng.rnd(), 1.0 / (n_exp + 1.0)); double phv = rng.rnd() * 2.0 * kPi;
             return polar_to_cart(F, ph, c, sqrt(1 - c * c), phv, reverse);
+        }
+        if constexpr ((FEAT & FEAT_AMSRC) != 0) {
+            if (type == MFG_DES_ANGLEMAP) {   // Particle.cpp:599-621: the map holds incident angles
+                const AmSrc A = am_view(P.am_src + ph.am_src_off);
+                double th, ov;
+                int tlo;
+                am_generate_theta(A, rng.rnd(), th, tlo, ov);
+                double phv = am_generate_phi(A, tlo, ov, rng.rnd());
+                double t2 = kPi - th;
+                return polar_to_cart(F, ph, cos(t2), sin(t2), phv, false);
+            }
         }
         double a = rng.rnd(); double phv = rng.rnd() * 2.0 * kPi;
         return polar_to_cart(F, ph, sqrt(a), sqrt(1 - a), phv, reverse);

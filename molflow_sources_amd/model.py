@@ -88,6 +88,7 @@ class Model:
     params: list = field(default_factory=list)
     moments: list = field(default_factory=list)   # (time, window) pairs
     og_maps: dict = field(default_factory=dict)   # facet -> 2D array (h, w) Pa*m3/s
+    am_maps: dict = field(default_factory=dict)   # facet -> recorded angle map counts (DES_ANGLEMAP)
     name: str = "model"
 
     def __post_init__(self):
@@ -164,6 +165,15 @@ class Model:
                 vals.append(np.asarray(m, dtype=np.float64).reshape(-1))
         ogo = arr(ogo, np.uint64)
         ogv = arr(np.concatenate(vals) if vals else np.zeros(1), np.float64)
+        amo = np.zeros(self.n_facets + 1, dtype=np.uint64)
+        amv = []
+        for f in range(self.n_facets):
+            a = self.am_maps.get(f)
+            amo[f + 1] = amo[f] + (0 if a is None else int(np.asarray(a).size))
+            if a is not None:
+                amv.append(np.asarray(a, dtype=np.uint64).reshape(-1))
+        amo = arr(amo, np.uint64)
+        amv = arr(np.concatenate(amv) if amv else np.zeros(1), np.uint64)
         settings = Settings()
         C.memmove(C.byref(settings), C.byref(self.settings), C.sizeof(Settings))
         keep.append(settings)
@@ -194,5 +204,8 @@ class Model:
         if self.og_maps:
             v.og_map_offset = ptr(ogo, C.c_uint64)
             v.og_map_values = ptr(ogv, C.c_double)
+        if self.am_maps:
+            v.am_pdf_offset = ptr(amo, C.c_uint64)
+            v.am_pdf = ptr(amv, C.c_uint64)
         self._keep = keep
         return v

@@ -254,6 +254,23 @@ std::vector<XY> integrated_desorption(const XY* par, size_t np, bool logx, bool 
 }
 
 // ------------------------------------------------------------------ geometry
+// DES_ANGLEMAP source distribution of one facet, built from its recorded
+// incident-angle map like InitializeAngleMap (src/Simulation/
+// MolflowSimFacet.cpp:92-222). Lines of the lower and higher theta parts
+// are concatenated: line t < L is lower line t, line t >= L is higher line
+// t - L.
+struct AngleSource {
+    int W = 0, L = 0, H = 0;
+    double limit = 0;
+    std::vector<uint64_t> pdf;        // (L+H)*W recorded counts
+    std::vector<uint64_t> line_sum;   // phi line sums (phi_pdfsums_*Theta)
+    std::vector<double> theta_cdf;    // CDF at line midpoints (theta_CDF_lower/higher)
+    std::vector<double> phi_cdf;      // (L+H)*W midpoint CDFs (phi_CDFs_*Theta)
+    std::vector<double> phi_pdf;      // (L+H)*W normalised pdfs (phi_pdfs_*Theta)
+    uint64_t sum_lower = 0, sum_higher = 0;   // theta_CDFsum_lower / _higher
+    double lower_ratio = 0;                   // thetaLowerRatio
+};
+
 struct Facet {
     mfg_facet_desc d;
     V3 O, U, V, N, nU, nV, center;
@@ -267,6 +284,7 @@ struct Facet {
     // source table
     std::vector<double> og_cdf;  // cumulative outgassing map (Pa*m3/s)
     int id_index = -1;
+    AngleSource am_src;
 };
 
 bool in_poly(const std::vector<double>& p, double u, double v) {
@@ -280,6 +298,202 @@ bool in_poly(const std::vector<double>& p, double u, double v) {
         }
     }
     return inside;
+}
+
+void build_angle_source(Facet& F, const std::vector<uint64_t>& pdf) {
+    AngleSource& A = F.am_src;
+    A.W = (int)F.d.am_phi_width; A.L = (int)F.d.am_theta_lower_res; A.H = (int)F.d.am_theta_higher_res;
+    A.limit = F.d.am_theta_limit;
+    const int W = A.W, L = A.L, H = A.H, T = L + H;
+    if (W <= 0 || T <= 0) throw std::runtime_error("DES_ANGLEMAP facet with an empty angle map shape");
+    if (pdf.size() != (size_t)T * W) throw std::runtime_error("DES_ANGLEMAP map size does not match its shape");
+    A.pdf = pdf;
+    A.line_sum.assign(T, 0);
+    A.theta_cdf.assign(T, 0.0);
+    A.phi_cdf.assign((size_t)T * W, 0.0);
+    A.phi_pdf.assign((size_t)T * W, 0.0);
+    // first pass: phi line sums, lower and total sums
+    A.sum_lower = 0;
+    for (int t = 0; t < L; ++t) {
+        for (int p = 0; p < W; ++p) A.line_sum[t] += pdf[(size_t)t * W + p];
+        A.sum_lower += A.line_sum[t];
+    }
+    A.sum_higher = A.sum_lower;   // the higher sum includes the lower part
+    for (int t = L; t < T; ++t) {
+        for (int p = 0; p < W; ++p) A.line_sum[t] += pdf[(size_t)t * W + p];
+        A.sum_higher += A.line_sum[t];
+    }
+    if (A.sum_higher == 0) throw std::runtime_error("DES_ANGLEMAP facet with an all-zero angle map");
+    A.lower_ratio = (double)A.sum_lower / (double)A.sum_higher;
+    // second pass: midpoint CDFs and normalised phi pdfs
+    const double tn = 1.0 / (double)A.sum_higher;
+    for (int t = 0; t < T; ++t) {
+        const bool first_of_part = (t == 0) || (t == L);
+        if (t == 0) A.theta_cdf[t] = 0.5 * (double)A.line_sum[0] * tn;
+        else if (t == L) A.theta_cdf[t] = A.lower_ratio + 0.5 * (double)A.line_sum[L] * tn;
+        else A.theta_cdf[t] = A.theta_cdf[t - 1] + (double)(A.line_sum[t - 1] + A.line_sum[t]) * 0.5 * tn;
+        (void)first_of_part;
+        const double pn = 1.0 / (double)A.line_sum[t];
+        for (int p = 0; p < W; ++p) {
+            const size_t i = (size_t)t * W + p;
+            if (A.line_sum[t] == 0) {   // empty line: uniform phi
+                A.phi_cdf[i] = (0.5 + (double)p) / (double)W;
+                A.phi_pdf[i] = 1.0 / (double)W;
+            } else {
+                if (p == 0) A.phi_cdf[i] = 0.5 * (double)pdf[(size_t)t * W] * pn;
+                else A.phi_cdf[i] = A.phi_cdf[i - 1] + (double)(pdf[i - 1] + pdf[i]) * 0.5 * pn;
+                A.phi_pdf[i] = (double)pdf[i] * pn;
+            }
+        }
+    }
+}
+
+// Helpers whose reference versions live in the absent shared library,
+// restated from their documented roles:
+//   lower_index: index of the last CDF value <= key, -1 below the first;
+//   weighed_lower_index: the same on the line (1-w)*A + w*B;
+//   weigh(a, b, w) = a + (b - a) * w; IDX: circular index.
+int lower_index(double key, const double* A, int n) {
+    return (int)(std::upper_bound(A, A + n, key) - A) - 1;
+}
+double weigh(double a, double b, double w) { return a + (b - a) * w; }
+int weighed_lower_index(double key, double w, const double* A, const double* B, int n) {
+    int lo = 0, hi = n;   // first index whose weighed value is > key
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (weigh(A[mid], B[mid], w) <= key) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+int circ(int i, int n) { return ((i % n) + n) % n; }
+
+// GetTheta / GetPhi (AnglemapGeneration.cpp): bin index -> angle
+double am_theta(const AngleSource& A, double ti) {
+    if ((size_t)ti < (size_t)A.L) return A.limit * ti / (double)A.L;
+    return A.limit + (kPi / 2.0 - A.limit) * (ti - (double)A.L) / (double)A.H;
+}
+double am_phi(const AngleSource& A, double pi_) {
+    double w = (double)A.W;
+    double c = (pi_ < w) ? pi_ : pi_ - w;
+    return -kPi + 2.0 * kPi * c / w;
+}
+// GetPhiNormalizedPdfValue
+double am_phi_pdf(const AngleSource& A, double ti, int plo) {
+    const int W = A.W, T = A.L + A.H;
+    const size_t p = (size_t)circ(plo, W);
+    if (ti < 0.5) return A.phi_pdf[p];
+    if (ti > (double)T - 0.5) return A.phi_pdf[(size_t)(T - 1) * W + p];
+    size_t tl = (size_t)(ti - 0.5);
+    double ov = ti - 0.5 - (double)tl;
+    return weigh(A.phi_pdf[tl * W + p], A.phi_pdf[(tl + 1) * W + p], ov);
+}
+// GetPhiCDFValue
+double am_phi_cdf(const AngleSource& A, double ti, int plo) {
+    const int W = A.W, L = A.L, T = A.L + A.H;
+    auto line = [&](size_t t) { return (plo < W) ? A.phi_cdf[t * W + plo] : 1.0 + A.phi_cdf[t * W]; };
+    if (ti < (double)L) {
+        if (ti < 0.5) return line(0);
+        if (ti > (double)L - 0.5) return line((size_t)(L - 1));
+    } else {
+        if (ti < (double)L + 0.5) return line((size_t)L);
+        if (ti > (double)T - 0.5) return line((size_t)(T - 1));
+    }
+    size_t tl = (size_t)(ti - 0.5);
+    double ov = ti - 0.5 - (double)tl;
+    return weigh(line(tl), line(tl + 1), ov);
+}
+
+// GenerateThetaFromAngleMap: theta, lower line index (-1 .. T-1), overshoot
+void am_generate_theta(const AngleSource& A, double r, double& theta, int& tlo, double& ov) {
+    const int L = A.L, H = A.H;
+    const double* cdf = A.theta_cdf.data();
+    auto quad = [&](int t, double c) {   // linear pdf between line midpoints t and t+1
+        double step = am_theta(A, (double)t + 1.5) - am_theta(A, (double)t + 0.5);
+        double b = (double)A.line_sum[t] / (double)A.sum_higher / step;
+        double a = 0.5 * ((double)A.line_sum[t + 1] - (double)A.line_sum[t]) / (double)A.sum_higher / (step * step);
+        double dy = r - c;
+        double dx = (-b + std::sqrt(b * b + 4 * a * dy)) / (2 * a);
+        return dx / step;
+    };
+    if (r < A.lower_ratio) {
+        tlo = lower_index(r, cdf, L);
+        if (tlo == -1) {
+            ov = 0.5 * r / cdf[0];
+            theta = am_theta(A, ov);
+        } else if (tlo == L - 1) {
+            ov = 0.5 * (r - cdf[tlo]) / (A.lower_ratio - cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        } else {
+            if (A.line_sum[tlo] == A.line_sum[tlo + 1]) ov = (r - cdf[tlo]) / (cdf[tlo + 1] - cdf[tlo]);
+            else ov = quad(tlo, cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        }
+    } else {
+        tlo = L + lower_index(r, cdf + L, H);
+        if (tlo == L - 1) {
+            ov = 0.5 * (r - A.lower_ratio) / (cdf[L] - A.lower_ratio);
+            theta = am_theta(A, (double)L + ov);
+        } else if (tlo == L + H - 1) {
+            ov = 0.5 * (r - cdf[tlo]) / (1.0 - cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        } else {
+            if (A.line_sum[tlo] == A.line_sum[tlo + 1]) ov = (r - cdf[tlo]) / (cdf[tlo + 1] - cdf[tlo]);
+            else ov = quad(tlo, cdf[tlo]);
+            theta = am_theta(A, (double)tlo + 0.5 + ov);
+        }
+    }
+}
+
+// GeneratePhiFromAngleMap
+double am_generate_phi(const AngleSource& A, int tlo, double ov, double r) {
+    const int W = A.W, L = A.L, T = A.L + A.H;
+    if (W == 1) return -kPi + 2.0 * kPi * r;
+    int plo;
+    double w;
+    const double* C = A.phi_cdf.data();
+    auto edge = [&](int t) {   // a single line, shifted by its first midpoint value
+        r += C[(size_t)t * W];
+        plo = lower_index(r, C + (size_t)t * W, W);
+        w = ov;
+    };
+    auto between = [&](int t) {   // weighed between lines t and t+1
+        double div = (double)A.line_sum[t] * (1.0 - ov) + (double)A.line_sum[t + 1] * ov;
+        w = div > 0.0 ? (ov * (double)A.line_sum[t + 1]) / div : ov;
+        r += weigh(C[(size_t)t * W], C[(size_t)(t + 1) * W], w);
+        plo = weighed_lower_index(r, w, C + (size_t)t * W, C + (size_t)(t + 1) * W, W);
+    };
+    if (tlo < L) {
+        if (tlo == -1) edge(0);
+        else if (tlo == L - 1) edge(tlo);
+        else between(tlo);
+    } else {
+        if (tlo == L - 1) edge(L);
+        else if (tlo == T - 1) edge(tlo);
+        else between(tlo);
+    }
+    const double ti = (double)tlo + 0.5 + w;
+    double phi, pov;
+    if (plo == -1) {
+        pov = 0.5 * r / am_phi_cdf(A, ti, 0);
+        phi = am_phi(A, pov);
+    } else {
+        double p0 = am_phi_pdf(A, ti, plo), p1 = am_phi_pdf(A, ti, plo + 1);
+        if (p0 == p1) {
+            double c0 = am_phi_cdf(A, ti, plo);
+            pov = (r - c0) / (am_phi_cdf(A, ti, plo + 1) - c0);
+        } else {
+            double step = 2.0 * kPi / (double)W;
+            double c = am_phi_cdf(A, ti, plo);
+            double b = p0 / step;
+            double a = 0.5 * (p1 - p0) / (step * step);
+            double dy = r - c;
+            double D = b * b + 4 * a * dy;
+            if (D < 0) pov = (r - c) / (am_phi_cdf(A, ti, circ(plo + 1, W)) - c);
+            else pov = ((-b + std::sqrt(b * b + 4 * a * dy)) / (2 * a)) / step;
+        }
+        phi = am_phi(A, (double)plo + 0.5 + pov);
+    }
+    return phi;
 }
 
 double poly_area(const std::vector<double>& p) {
@@ -556,7 +770,11 @@ void load_model(Model& M, const mfg_model_view* v) {
             for (uint64_t k = v->og_map_offset[f]; k < v->og_map_offset[f + 1]; ++k) { acc += v->og_map_values[k]; F.og_cdf.push_back(acc); }
             if (F.og_cdf.size() != (size_t)F.d.og_map_width * F.d.og_map_height) throw std::runtime_error("outgassing map size mismatch");
         } else F.d.use_outgassing_map = 0;
-        if (F.d.desorb_type == MFG_DES_ANGLEMAP) throw std::runtime_error("DES_ANGLEMAP desorption is not supported by this checker");
+        if (F.d.desorb_type == MFG_DES_ANGLEMAP) {
+            if (!v->am_pdf_offset || !v->am_pdf) throw std::runtime_error("DES_ANGLEMAP facet without a recorded angle map");
+            std::vector<uint64_t> pdf(v->am_pdf + v->am_pdf_offset[f], v->am_pdf + v->am_pdf_offset[f + 1]);
+            build_angle_source(F, pdf);
+        }
         // integrated desorption per distinct outgassing parameter
         if (F.d.outgassing_param >= 0) {
             int p = F.d.outgassing_param;
@@ -987,6 +1205,15 @@ struct Tracer {
         switch (F.d.desorb_type) {
             case MFG_DES_UNIFORM: { double c = rng.rnd(); double ph = rng.rnd() * 2.0 * kPi; dir = polar_to_cart(F, c, std::sqrt(1 - c * c), ph, reverse); break; }
             case MFG_DES_COSINE_N: { double c = std::pow(rng.rnd(), 1.0 / (F.d.desorb_exponent + 1.0)); double ph = rng.rnd() * 2.0 * kPi; dir = polar_to_cart(F, c, std::sqrt(1 - c * c), ph, reverse); break; }
+            case MFG_DES_ANGLEMAP: {   // Particle.cpp:599-621
+                double th, ov;
+                int tlo;
+                am_generate_theta(F.am_src, rng.rnd(), th, tlo, ov);
+                double ph = am_generate_phi(F.am_src, tlo, ov, rng.rnd());
+                double t2 = kPi - th;   // the map holds incident angles
+                dir = polar_to_cart(F, std::cos(t2), std::sin(t2), ph, false);
+                break;
+            }
             default: { double a = rng.rnd(); double ph = rng.rnd() * 2.0 * kPi; dir = polar_to_cart(F, std::sqrt(a), std::sqrt(1 - a), ph, reverse); break; }
         }
         structure = F.d.super_idx;

@@ -37,6 +37,16 @@ def model_from_fixture(name: str):
     m.params = [Parameter(p["name"], p["x"], p["y"], p["log_x"], p["log_y"]) for p in d["params"]]
     for content, window in d["user_moments"]:
         m.add_user_moments(content, window)
+    # recorded incident-angle maps that DES_ANGLEMAP facets desorb from; the
+    # stored map's shape replaces the facet's angle-map parameters, as the
+    # reference loader does (LoaderXML.cpp:1043-1049)
+    for f, am in d.get("angle_maps_in", {}).items():
+        f = int(f)
+        m.am_maps[f] = np.asarray(am["map"], dtype=np.uint64)
+        m.facets["am_phi_width"][f] = am["phi_width"]
+        m.facets["am_theta_limit"][f] = am["theta_limit"]
+        m.facets["am_theta_lower_res"][f] = am["lower"]
+        m.facets["am_theta_higher_res"][f] = am["higher"]
     return m, d
 
 

@@ -76,7 +76,8 @@ def test_first_hit_every_structure(gpu):
 # ---------------------------------------------------------------- event traces
 TRACE_MODELS = ["tube", "01-quick_pipe_profiles_textures_2sided", "02-timedependent_with_two_parameters",
                 "03b-anglemap_record_transparent_target", "05-three_structures_nonsquare_textures",
-                "07-volume_decay", "08-wall_sojourn_time", "B04-lr10_pipe_trans"]
+                "07-volume_decay", "08-wall_sojourn_time", "B04-lr10_pipe_trans", "04-anglemap_desorb",
+                "04b-anglemap_desorb-sticking_source"]
 
 
 @pytest.mark.parametrize("name", TRACE_MODELS)
@@ -184,7 +185,8 @@ COMPARE = [
     ("03b-anglemap_record_transparent_target", 1_000_000), ("05-three_structures_nonsquare_textures", 2_000_000),
     ("07-volume_decay", 200_000), ("08-wall_sojourn_time", 2_000_000), ("09-histograms", 4_000_000),
     ("B01-lr1000_pipe", 400_000), ("B02-lr10_pipe_tex", 4_000_000), ("B03-lr10_pipe_prof", 4_000_000),
-    ("B04-lr10_pipe_trans", 2_000_000),
+    ("B04-lr10_pipe_trans", 2_000_000), ("04-anglemap_desorb", 1_000_000),
+    ("04b-anglemap_desorb-sticking_source", 1_000_000),
 ]
 
 
@@ -241,3 +243,24 @@ def test_time_dependent_config3_moments(gpu):
     assert np.corrcoef(pg, pc)[0, 1] > 0.98
     # the steady-state slot is the sum over moments plus out-of-window hits
     assert g.facets["nb_mc_hit"][0].sum() >= g.facets["nb_mc_hit"][1:].sum()
+
+
+def test_angle_map_source_without_a_map_is_refused(gpu):
+    from molflow_sources_amd.simulation import MolflowGpuError
+    m, _ = model_from_fixture("04-anglemap_desorb")
+    m.am_maps = {}
+    with pytest.raises(MolflowGpuError, match="DES_ANGLEMAP"):
+        gpu.load(m)
+
+
+def test_angle_map_source_same_seed_tallies(gpu):
+    """DES_ANGLEMAP desorption draws the same directions as the oracle: the
+    same seed gives the same integer tallies."""
+    m, _ = model_from_fixture("04-anglemap_desorb")
+    n = 20_000
+    loaded(gpu, m, seed=31)
+    gpu.run(n, max_events=MAXEV)
+    g = gpu.read()
+    c = CpuModel(m).run(n, seed=31, max_events=MAXEV)
+    diff = np.abs(g.facets["nb_mc_hit"][0].astype(np.int64) - c.facets["nb_mc_hit"][0].astype(np.int64)).sum()
+    assert diff <= 1e-3 * c.glob[0]["nb_mc_hit"], diff

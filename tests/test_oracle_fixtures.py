@@ -71,6 +71,8 @@ COMPARE = [
     ("02b-timedependent_with_three_parameters", 24_000),
     ("03-anglemap_record", 80_000),
     ("03b-anglemap_record_transparent_target", 80_000),
+    ("04-anglemap_desorb", 80_000),
+    ("04b-anglemap_desorb-sticking_source", 80_000),
     ("05-three_structures_nonsquare_textures", 200_000),
     ("07-volume_decay", 16_000),
     ("08-wall_sojourn_time", 160_000),
@@ -216,10 +218,29 @@ def test_histograms_fixture09(oracle_lib):
     assert _chi2_gof(gm[601:], gh["Time"]) > 1e-3
 
 
-def test_angle_map_desorption_is_rejected(oracle_lib):
+def test_angle_map_desorption_needs_a_recorded_map(oracle_lib):
     m, _ = model_from_fixture("04-anglemap_desorb")
+    m.am_maps = {}
     with pytest.raises(ValueError, match="DES_ANGLEMAP"):
         CpuModel(m)
+
+
+@pytest.mark.parametrize("name", ["04-anglemap_desorb", "04b-anglemap_desorb-sticking_source"])
+def test_angle_map_source_absorption_split(oracle_lib, name):
+    """Per-facet absorption of the angle-map-desorbing fixtures: every
+    particle is absorbed once, so the split is multinomial (exact chi2)."""
+    m, d = model_from_fixture(name)
+    n = 200_000
+    t = CpuModel(m).run(n, seed=19, max_events=10_000_000)
+    S = StoredState(d)
+    p_ref = S.facets["nb_abs_equiv"][0] / S.glob[0]["nb_desorbed"]
+    obs = t.facets["nb_abs_equiv"][0]
+    keep = p_ref * n > 20
+    exp = p_ref[keep] * n
+    chi2 = float(np.sum((obs[keep] - exp) ** 2 / exp))
+    dof = int(keep.sum()) - 1
+    assert dof >= 2
+    assert chi2 < dof + 5 * math.sqrt(2 * dof), (chi2, dof)
 
 
 def test_fixture_files_hold_reference_values():

@@ -106,3 +106,33 @@ def test_low_flux_conserves_desorbed_weight(oracle_lib):
     t = CpuModel(m).run(n, seed=2)
     g = t.glob[0]
     assert abs(g["nb_abs_equiv"] / n - 1) < 1e-6
+
+
+def angle_map_source_model(W=20, L=20, H=20, limit=0.1, phi_mod=0.0):
+    """The box with facet 0 desorbing from a recorded incident-angle map
+    built from a cosine law in theta (counts per bin = integral of
+    cos(t) sin(t) dt) and a 1 + phi_mod * sin(phi) law in phi."""
+    m = box_model(A.DES_ANGLEMAP)
+    edges = np.concatenate([np.linspace(0, limit, L + 1), np.linspace(limit, math.pi / 2, H + 1)[1:]])
+    t_mass = 0.5 * (np.sin(edges[1:]) ** 2 - np.sin(edges[:-1]) ** 2)
+    pe = np.linspace(-math.pi, math.pi, W + 1)
+    p_mass = (pe[1:] - pe[:-1]) - phi_mod * (np.cos(pe[1:]) - np.cos(pe[:-1]))
+    p_mass /= p_mass.sum()
+    counts = np.round(1e9 * np.outer(t_mass, p_mass)).astype(np.uint64)
+    F = m.facets
+    F["am_phi_width"][0], F["am_theta_lower_res"][0], F["am_theta_higher_res"][0] = W, L, H
+    F["am_theta_limit"][0] = limit
+    m.am_maps[0] = counts.reshape(-1)
+    return m
+
+
+def test_angle_map_source_reproduces_the_recorded_law(oracle_lib):
+    th, ph, _ = emitted_angles(angle_map_source_model(phi_mod=0.5), 40000)
+    # the map holds incident angles: particles leave on the facet's back side
+    assert np.all(np.cos(th) < 0)
+    c = -np.cos(th)                      # cos of the recorded incident angle
+    assert stats.kstest(c, lambda x: x ** 2).pvalue > 1e-3
+    # phi law 1 + 0.5 sin(phi): E[sin phi] = 0.25, less the binning smoothing
+    s = np.sin(ph)
+    assert abs(s.mean() - 0.25) < 0.01 + 4 * s.std() / math.sqrt(len(s))
+    assert abs(np.cos(ph).mean()) < 4 * np.cos(ph).std() / math.sqrt(len(ph))
