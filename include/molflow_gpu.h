@@ -394,7 +394,8 @@ typedef struct mfg_tune_result {
     uint32_t waves;          /* chosen waves per SIMD */
     uint32_t block;          /* threads per block */
     uint32_t blocks_per_cu;  /* resident blocks per CU at that shape */
-    uint32_t reserved;
+    uint32_t n_hot;          /* facets whose slot-0 counters are summed per block in LDS
+                                (those with >= 0.5 % of the probes' counter updates) */
     double hits_per_s;       /* probe throughput of the choice */
 } mfg_tune_result;
 int mfg_autotune(mfg_context* ctx, uint64_t n_probe, mfg_tune_result* out);

@@ -559,6 +559,7 @@ void prepare(HostModel& M, const mfg_model_view* v) {
         if (d.count_trans) fl |= FF_COUNT_TRANS;
         if (d.count_dir) fl |= FF_COUNT_DIR;
         ph.am_src_off = -1;
+        ph.hot_idx = -1;
         if (d.desorb_type == MFG_DES_ANGLEMAP) {
             if (!v->am_pdf_offset || !v->am_pdf || v->am_pdf_offset[f + 1] == v->am_pdf_offset[f])
                 throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": DES_ANGLEMAP without a recorded angle map");
@@ -732,6 +733,9 @@ struct mfg_context {
     DevBuf<int32_t> src_facet, roots;
     DevBuf<LeafFacet> leaves;
     DevBuf<LeafUV> leaf_uv;
+    std::vector<DPhys> h_phys;           // host copy of the uploaded physics records
+    std::vector<int32_t> hot;            // hot facets (global index), their counters summed in LDS
+    DevBuf<uint64_t> hot_fc;
     std::vector<DFacet> h_facets;        // host copies of the concatenated facets and polygons
     std::vector<double> h_poly;
     DevBuf<uint32_t> guides;
@@ -800,6 +804,25 @@ void reset_graph(mfg_context* c) {
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     c->graph_key.clear();
+}
+
+// Marks the hot facets in the physics records and stores the index of each
+// one's slot-0 counter record.
+void set_hot_facets(mfg_context* c, const std::vector<int32_t>& hot, bool upload) {
+    for (int32_t f : c->hot) c->h_phys[(size_t)f].hot_idx = -1;
+    c->hot = hot;
+    std::vector<uint64_t> idx;
+    for (size_t k = 0; k < hot.size(); ++k) {
+        DPhys& ph = c->h_phys[(size_t)hot[k]];
+        ph.hot_idx = (int32_t)k;
+        size_t vi = 0;
+        while (vi + 1 < c->dvars.size() && c->dvars[vi + 1].facet_base <= (uint32_t)hot[k]) ++vi;
+        idx.push_back(c->dvars[vi].fc_base + (uint64_t)(uint32_t)ph.local_id);
+    }
+    c->hot_fc.upload(idx, c->stream);
+    if (upload) c->phys.upload(c->h_phys, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    reset_graph(c);
 }
 
 void upload_models(mfg_context* c) {
@@ -906,7 +929,27 @@ void upload_models(mfg_context* c) {
         c->layouts.push_back(std::move(L));
     }
     hipStream_t s = c->stream;
-    c->facets.upload(facets, s); c->phys.upload(phys, s); c->poly.upload(poly, s);
+    c->h_phys = phys;
+    // default hot facets of each variant: its sources and absorbers when
+    // there are only a few of them (the autotuner refines the choice from
+    // measured counts)
+    std::vector<int32_t> hot;
+    for (size_t vi = 0; vi < c->models.size(); ++vi) {
+        const HostModel& M = *c->models[vi];
+        const uint32_t fb = c->dvars[vi].facet_base;
+        std::vector<int32_t> src(M.src_facet.begin(), M.src_facet.end()), ab;
+        for (size_t f = 0; f < M.phys.size(); ++f)
+            if (M.phys[f].sticking > 0.0 || M.phys[f].sticking_param >= 0) ab.push_back((int32_t)f);
+        std::vector<int32_t> pick;
+        if (src.size() <= 4) pick.insert(pick.end(), src.begin(), src.end());
+        if (ab.size() <= 8) pick.insert(pick.end(), ab.begin(), ab.end());
+        std::sort(pick.begin(), pick.end());
+        pick.erase(std::unique(pick.begin(), pick.end()), pick.end());
+        for (int32_t f : pick) if (hot.size() < MFG_MAX_HOT) hot.push_back(f + (int32_t)fb);
+    }
+    c->hot.clear();
+    set_hot_facets(c, hot, false);
+    c->facets.upload(facets, s); c->phys.upload(c->h_phys, s); c->poly.upload(poly, s);
     c->h_facets = facets;
     c->h_poly = poly;
     c->src_cum.upload(src_cum, s); c->src_facet.upload(src_facet, s); c->og_cdf.upload(og_cdf, s);
@@ -1048,6 +1091,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size(); p.stack_size = c->stack_size;
     p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p; p.am_src = c->am_src.p;
+    p.hot_fc = c->hot_fc.p; p.n_hot = (uint32_t)c->hot.size();
     p.guides = c->guides.p; p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
     p.mom_start = c->mom_start.p; p.mom_end = c->mom_end.p;
     p.fc = c->fc.p; p.force = c->force.p; p.tex = c->tex.p; p.tex_inc = c->tex_inc.p; p.dir = c->dir.p;
@@ -1078,7 +1122,7 @@ void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& wa
     uint32_t per_cu = c->blocks_per_cu;
     if (per_cu == 0) {
         // fill every CU to its occupancy limit (persistent launch)
-        size_t lds = (size_t)block * c->stack_size * sizeof(int32_t);
+        size_t lds = (size_t)block * c->stack_size * sizeof(int32_t) + c->hot.size() * 8 * sizeof(double);
         int occ = mfg_trace_occupancy(feats, waves, block, lds);
         per_cu = (uint32_t)std::max(1, occ);
     }
@@ -1139,6 +1183,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
     key.push_back((uint64_t)waves);
     key.push_back((uint64_t)(uintptr_t)p.carry); key.push_back(p.carry_keep);
+    key.push_back((uint64_t)(uintptr_t)p.hot_fc); key.push_back(p.n_hot);
     // the events bracket the trace kernel alone (eager path) or the whole
     // graph (replay path)
     auto enqueue = [&](bool events) {
@@ -1594,6 +1639,31 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
                 }
             }
         c->waves = (int)r.waves; c->block = r.block; c->blocks_per_cu = 0;
+        // hot facets: those that took at least 1/2 % of the probes' slot-0
+        // counter updates (hits + desorptions), most frequent first
+        {
+            std::vector<std::pair<uint64_t, int32_t>> ev;
+            uint64_t total = 0;
+            for (size_t vi = 0; vi < c->dvars.size(); ++vi) {
+                const DVariant& dv = c->dvars[vi];
+                std::vector<mfg_facet_counters> h(dv.n_facets);
+                HIPCHK(hipMemcpy(h.data(), c->fc.p + dv.fc_base, dv.n_facets * sizeof(mfg_facet_counters), hipMemcpyDeviceToHost));
+                for (uint32_t f = 0; f < dv.n_facets; ++f) {
+                    uint64_t e = h[f].nb_mc_hit + h[f].nb_desorbed;
+                    total += e;
+                    if (e) ev.push_back({e, (int32_t)(dv.facet_base + f)});
+                }
+            }
+            std::sort(ev.begin(), ev.end(), [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
+            std::vector<int32_t> hot;
+            for (auto& e : ev) {
+                if (hot.size() >= MFG_MAX_HOT || e.first * 200 < total) break;
+                hot.push_back(e.second);
+            }
+            std::sort(hot.begin(), hot.end());
+            set_hot_facets(c, hot, true);
+            r.n_hot = (uint32_t)hot.size();
+        }
         c->pid_base = save_pid;
         zero_tallies(c);
         reset_graph(c);
@@ -1613,13 +1683,13 @@ int mfg_adjoint_setup(mfg_context* c, const int32_t* group_of_facet, uint32_t G,
         if (G == 0 || G > MFG_MAX_GROUPS) throw Err(MFG_ERR_LIMIT, "number of sticking groups must be 1..64");
         if (!group_of_facet || !targets || T == 0) throw Err(MFG_ERR_INVALID, "null adjoint arrays");
         uint32_t nF = (uint32_t)M.phys.size();
-        std::vector<DPhys> phys(M.phys);
+        std::vector<DPhys>& phys = c->h_phys;   // one variant: global = local facet index
         for (uint32_t f = 0; f < nF; ++f) {
             int32_t g = group_of_facet[f];
             if (g >= (int32_t)G) throw Err(MFG_ERR_INVALID, "group index out of range");
             phys[f].adj_group = g < 0 ? -1 : g;
             phys[f].adj_target = -1;
-            if (g >= 0) phys[f].sticking_param = -1;
+            phys[f].sticking_param = g >= 0 ? -1 : M.phys[f].sticking_param;
         }
         for (uint32_t t = 0; t < T; ++t) {
             if (targets[t] < 0 || (uint32_t)targets[t] >= nF) throw Err(MFG_ERR_INVALID, "target facet out of range");

@@ -21,6 +21,7 @@
 #define MFG_MAX_STACK 64
 #define MFG_MAX_GROUPS 64
 #define MFG_MAXWELL_GUIDE 128
+#define MFG_MAX_HOT 32           // facets whose counters are privatised per block
 
 // feature bits of a trace-kernel instantiation / of a loaded model
 enum : uint32_t {
@@ -109,7 +110,8 @@ struct alignas(16) DPhys {
     int32_t desorb_type;
     int32_t pad_;
     int64_t am_src_off;             // DES_ANGLEMAP source table in the pool (global), -1
-    int64_t pad2_;
+    int32_t hot_idx;                // slot in the per-block hot-facet counter table, -1
+    int32_t pad2_;
 };
 
 struct alignas(16) BvhNode {
@@ -237,6 +239,10 @@ struct TraceParams {
     // [W, L, H, theta limit, lower ratio, total count, line sums (T),
     //  theta CDF (T), phi CDFs (T*W), phi pdfs (T*W)], T = L + H
     const double* am_src;
+    // hot facets: their slot-0 counter updates are summed in LDS per block
+    // and added to the global records once, when the block ends
+    const uint64_t* hot_fc;           // index into fc of each hot facet's slot-0 record
+    uint32_t n_hot;
     // tallies
     mfg_facet_counters* fc;
     double* force;

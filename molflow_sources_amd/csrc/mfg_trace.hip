@@ -559,6 +559,7 @@ struct Tracer {
     uint32_t acc_leak, acc_decay;
     uint32_t tr_n;
     uint32_t* adj_cnt;
+    double* hot;                      // LDS table of the hot facets' slot-0 counters (8 words each)
     // pending slot-0 facet counter update of this lane
     int32_t pend_f;
     uint32_t pend_hits;
@@ -571,6 +572,7 @@ struct Tracer {
         tr_n = 0;
         adj_cnt = nullptr;
         var = nullptr;
+        hot = nullptr;
         clear_pending();
     }
 
@@ -673,6 +675,16 @@ struct Tracer {
     __device__ void flush_pending_direct() {
         if (pend_f < 0) return;
         const DPhys& ph = P.phys[pend_f];
+        if (ph.hot_idx >= 0) {
+            double* h = hot + 8 * ph.hot_idx;   // same field order as mfg_facet_counters
+            if (pend_hits) { atomicAdd(reinterpret_cast<unsigned long long*>(h), (unsigned long long)pend_hits); atomic_addd(h + 1, pend_he); }
+            if (pend_abs != 0.0) atomic_addd(h + 3, pend_abs);
+            if (pend_s1v != 0.0) atomic_addd(h + 4, pend_s1v);
+            if (pend_svo != 0.0) atomic_addd(h + 5, pend_svo);
+            if (pend_s1pv != 0.0) atomic_addd(h + 6, pend_s1pv);
+            clear_pending();
+            return;
+        }
         mfg_facet_counters* cc = P.fc + var->fc_base + (uint32_t)ph.local_id;
         if (pend_hits) { atomic_addu(&cc->nb_mc_hit, pend_hits); atomic_addd(&cc->nb_hit_equiv, pend_he); }
         if (pend_abs != 0.0) atomic_addd(&cc->nb_abs_equiv, pend_abs);
@@ -1402,6 +1414,11 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     const uint32_t lane = __lane_id();
     int32_t* stk = lds_stack + threadIdx.x;
     Tracer<FEAT> T(P, stk, blockDim.x);
+    // hot-facet counters after the stacks (stack bytes are a multiple of 8)
+    double* hot = reinterpret_cast<double*>(lds_stack + (size_t)blockDim.x * P.stack_size);
+    for (uint32_t i = threadIdx.x; i < 8u * P.n_hot; i += blockDim.x) hot[i] = 0.0;
+    __syncthreads();
+    T.hot = hot;
     if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
         T.adj_cnt = P.adj_counts + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * P.adj_groups;
     }
@@ -1485,11 +1502,19 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
                 uint64_t gmask = __ballot(grp);
                 double s1 = wave_sum(grp ? w1 : 0.0), s2 = wave_sum(grp ? w2 : 0.0), s3 = wave_sum(grp ? w3 : 0.0);
                 if (grp && lane == sl) {
-                    mfg_facet_counters* c = T.fc(*ph, 0);
-                    atomic_addu(&c->nb_desorbed, (unsigned long long)__popcll(gmask));
-                    atomic_addd(&c->sum_1_per_ort_velocity, s1);
-                    atomic_addd(&c->sum_v_ort, s2);
-                    atomic_addd(&c->sum_1_per_velocity, s3);
+                    if (ph->hot_idx >= 0) {
+                        double* h = hot + 8 * ph->hot_idx;
+                        atomicAdd(reinterpret_cast<unsigned long long*>(h + 2), (unsigned long long)__popcll(gmask));
+                        atomic_addd(h + 4, s1);
+                        atomic_addd(h + 5, s2);
+                        atomic_addd(h + 6, s3);
+                    } else {
+                        mfg_facet_counters* c = T.fc(*ph, 0);
+                        atomic_addu(&c->nb_desorbed, (unsigned long long)__popcll(gmask));
+                        atomic_addd(&c->sum_1_per_ort_velocity, s1);
+                        atomic_addd(&c->sum_v_ort, s2);
+                        atomic_addd(&c->sum_1_per_velocity, s3);
+                    }
                 }
                 if (started && !grp) {
                     mfg_facet_counters* c = T.fc(*ph, 0);
@@ -1562,6 +1587,20 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         }
     } else {
         T.flush_lane();
+    }
+    // every wave of the block has left the loop: add the block's hot-facet
+    // sums to the global records
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 8u * P.n_hot; i += blockDim.x) {
+        const uint32_t k = i >> 3, fld = i & 7u;
+        if (fld == 7u) continue;
+        double* g = reinterpret_cast<double*>(P.fc + P.hot_fc[k]) + fld;
+        if (fld == 0u || fld == 2u) {
+            unsigned long long v = *reinterpret_cast<unsigned long long*>(hot + i);
+            if (v) atomic_addu(reinterpret_cast<unsigned long long*>(g), v);
+        } else if (hot[i] != 0.0) {
+            atomic_addd(g, hot[i]);
+        }
     }
 }
 
@@ -1670,7 +1709,7 @@ int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds) {
 }
 
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream) {
-    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t);
+    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t) + (size_t)p.n_hot * 8 * sizeof(double);
     TraceKernel k = kernel_for(mfg_trace_kernel_features(features), waves);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
