@@ -382,7 +382,7 @@ int mfg_set_carry(mfg_context* ctx, int enable);
 /* Number of particles parked on the device after the last run. */
 int mfg_in_flight(const mfg_context* ctx, uint64_t* n);
 
-/* Occupancy bound of the trace kernel: waves per SIMD (1..4; 0 = default).
+/* Occupancy bound of the trace kernel: waves per SIMD (1..5; 0 = default).
  * Higher bounds cap registers (cold state spills) to hide more latency. */
 int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);
 

@@ -724,7 +724,8 @@ struct mfg_context {
     uint64_t seed = 42;
     uint32_t features = 0;
     bool loaded = false, accel = false;
-    uint32_t stack_size = 0;
+    uint32_t stack_size = 0;             // traversal stack entries a walk may need
+    DevBuf<int32_t> stack_spill;         // per-lane entries beyond the LDS part
     int bvh_width = 4, leaf_size = 2;
     // device data
     DevBuf<DFacet> facets;
@@ -1089,7 +1090,10 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     TraceParams p{};
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
     p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
-    p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size(); p.stack_size = c->stack_size;
+    p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size();
+    p.stack_size = std::min<uint32_t>(c->stack_size, MFG_LDS_STACK);
+    p.stack_total = c->stack_size;
+    p.stack_spill = p.stack_total > p.stack_size ? c->stack_spill.p : nullptr;
     p.src_cum = c->src_cum.p; p.src_facet = c->src_facet.p; p.og_cdf = c->og_cdf.p; p.am_src = c->am_src.p;
     p.hot_fc = c->hot_fc.p; p.n_hot = (uint32_t)c->hot.size();
     p.guides = c->guides.p; p.tab_x = c->tab_x.p; p.tab_y = c->tab_y.p; p.params = c->params.p; p.ids = c->ids.p;
@@ -1122,7 +1126,8 @@ void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& wa
     uint32_t per_cu = c->blocks_per_cu;
     if (per_cu == 0) {
         // fill every CU to its occupancy limit (persistent launch)
-        size_t lds = (size_t)block * c->stack_size * sizeof(int32_t) + c->hot.size() * 8 * sizeof(double);
+        size_t lds = (size_t)block * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
+                     c->hot.size() * 8 * sizeof(double);
         int occ = mfg_trace_occupancy(feats, waves, block, lds);
         per_cu = (uint32_t)std::max(1, occ);
     }
@@ -1173,6 +1178,10 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
         if (c->adj_counts.n < need) c->adj_counts.alloc(need);
     }
     if (c->carry_on) ensure_carry(c, (uint64_t)grid * (uint64_t)block);
+    if (c->stack_size > MFG_LDS_STACK) {
+        size_t need = (size_t)grid * block * (c->stack_size - MFG_LDS_STACK);
+        if (c->stack_spill.n < need) { c->stack_spill.alloc(need); reset_graph(c); }
+    }
     // read the global counters before the run to report this run's hits
     std::vector<DGlobal> before(nv);
     hipStream_t s = c->stream;
@@ -1603,7 +1612,7 @@ int mfg_in_flight(const mfg_context* c, uint64_t* n) {
 
 int mfg_set_kernel_variant(mfg_context* c, uint32_t waves) {
     return guarded(c, [&] {
-        if (waves > 4) throw Err(MFG_ERR_INVALID, "waves per SIMD must be 0..4");
+        if (waves > 5) throw Err(MFG_ERR_INVALID, "waves per SIMD must be 0..5");
         c->waves = (int)waves;
         reset_graph(c);
     });
@@ -1614,8 +1623,8 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
         if (!c->accel) throw Err(MFG_ERR_STATE, "acceleration structure not built");
         if (n_probe == 0) throw Err(MFG_ERR_INVALID, "probe size must be positive");
         uint32_t feats = c->features | (c->trace_n ? FEAT_TRACE : 0u) | (c->adj_groups ? FEAT_ADJOINT : 0u);
-        int wl[4];
-        int nw = mfg_trace_variants(feats, wl, 4);
+        int wl[8];
+        int nw = mfg_trace_variants(feats, wl, 8);
         const uint32_t blocks[] = {128, 256};
         size_t nv = c->dvars.size();
         std::vector<uint64_t> nd(nv, std::max<uint64_t>(1, n_probe / nv)), seeds(nv, c->seed);

@@ -19,6 +19,7 @@
 
 #define MFG_BVH_WIDTH 4
 #define MFG_MAX_STACK 64
+#define MFG_LDS_STACK 16         // traversal stack entries per lane kept in LDS by the tracer
 #define MFG_MAX_GROUPS 64
 #define MFG_MAXWELL_GUIDE 128
 #define MFG_MAX_HOT 32           // facets whose counters are privatised per block
@@ -225,6 +226,8 @@ struct TraceParams {
     const uint64_t* var_prefix;       // n_variants+1 prefix of this run's desorptions
     uint32_t n_variants;
     uint32_t stack_size;              // LDS stack entries per lane
+    uint32_t stack_total;             // entries a walk may need; the ones past stack_size
+    int32_t* stack_spill;             // live in this per-lane global area (lane-major)
     const double* src_cum;            // cumulative source weights
     const int32_t* src_facet;
     const double* og_cdf;

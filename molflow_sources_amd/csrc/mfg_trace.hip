@@ -459,12 +459,33 @@ __device__ __forceinline__ double td_opacity(const TraceParams& P, const DVarian
     return interp_y(time, XYRef{P.tab_x + tb.off, P.tab_y + tb.off, tb.n}, tb.log_x, tb.log_y, false);
 }
 
+// Traversal stack of one lane: the first `lds_cap` entries in LDS (laid out
+// [entry][lane]), deeper entries (rare) in a per-lane global area.
+struct LaneStack {
+    int32_t* lds;
+    uint32_t stride, lds_cap, cap;
+    int32_t* spill;
+    __device__ __forceinline__ void push(uint32_t& sp, int32_t v) const {
+        if (sp < lds_cap) {
+            lds[(size_t)sp * stride] = v;
+            ++sp;
+        } else if (sp < cap) {
+            spill[sp - lds_cap] = v;
+            ++sp;
+        }
+    }
+    __device__ __forceinline__ int32_t pop(uint32_t& sp) const {
+        --sp;
+        return sp < lds_cap ? lds[(size_t)sp * stride] : spill[sp - lds_cap];
+    }
+};
+
 // Nearest hard hit below tmax. With TRANSP, non-opaque facets are hard with
 // probability `opacity` (coin per event and facet); opaque_only ignores
 // opacity (fixed ray batches).
 template <bool TRANSP>
 __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int32_t skip, double tmax,
-                           int32_t* __restrict__ stk, uint32_t stride, uint32_t stack_size, const Rng* rng,
+                           const LaneStack& S, const Rng* rng,
                            uint32_t event, const DVariant* var, double time, bool opaque_only) {
     HitRec h{-1, tmax, 0.0, 0.0};
     if (root < 0) return h;
@@ -526,15 +547,14 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         MFG_CSWAP(k1, c1, k2, c2)
 #undef MFG_CSWAP
         if (c0 >= 0) {
-            if (c3 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c3; ++sp; }
-            if (c2 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c2; ++sp; }
-            if (c1 >= 0 && sp < stack_size) { stk[(size_t)sp * stride] = c1; ++sp; }
+            if (c3 >= 0) S.push(sp, c3);
+            if (c2 >= 0) S.push(sp, c2);
+            if (c1 >= 0) S.push(sp, c1);
             node = c0;
             continue;
         }
         if (sp == 0) break;
-        --sp;
-        node = stk[(size_t)sp * stride];
+        node = S.pop(sp);
     }
     return h;
 }
@@ -543,8 +563,7 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
 template <uint32_t FEAT>
 struct Tracer {
     const TraceParams& P;
-    int32_t* stk;
-    uint32_t stride;
+    LaneStack S;
     // particle state (reference Particle.h:95-122)
     d3 pos, dir;
     double time, vel, ori, gen_time, dist, decay_t, scatter_path, hu, hv;
@@ -565,7 +584,7 @@ struct Tracer {
     uint32_t pend_hits;
     double pend_he, pend_abs, pend_s1v, pend_svo, pend_s1pv;
 
-    __device__ Tracer(const TraceParams& p, int32_t* s, uint32_t st) : P(p), stk(s), stride(st) {
+    __device__ Tracer(const TraceParams& p, const LaneStack& s) : P(p), S(s) {
         acc_hits = acc_des = 0;
         acc_hit_equiv = acc_abs_equiv = acc_dist_total = acc_dist_full = 0.0;
         acc_leak = acc_decay = 0;
@@ -1295,7 +1314,7 @@ struct Tracer {
         constexpr bool TR = (FEAT & FEAT_TRANSPARENT) != 0;
         ++nb_events;
         int32_t root = P.roots[var->root_base + (uint32_t)structure];
-        HitRec h = traverse<TR>(P, root, pos, dir, last_hit, 1e300, stk, stride, P.stack_size, &rng, nb_events, var,
+        HitRec h = traverse<TR>(P, root, pos, dir, last_hit, 1e300, S, &rng, nb_events, var,
                                 time, false);
         if (h.facet < 0) {  // leak
             ++acc_leak;
@@ -1331,7 +1350,7 @@ struct Tracer {
                         msk &= msk - 1u;
                         int32_t ch = c == 0 ? nd.ch.x : (c == 1 ? nd.ch.y : (c == 2 ? nd.ch.z : nd.ch.w));
                         if (ch >= 0) {
-                            if (sp < P.stack_size) { stk[(size_t)sp * stride] = ch; ++sp; }
+                            S.push(sp, ch);
                             continue;
                         }
                         uint32_t first = (uint32_t)(~ch), cnt = (nd.meta.x >> (8u * c)) & 0xffu;
@@ -1351,8 +1370,7 @@ struct Tracer {
                         }
                     }
                     if (sp == 0) break;
-                    --sp;
-                    node = stk[(size_t)sp * stride];
+                    node = S.pop(sp);
                 }
                 hu = save_u; hv = save_v;
             }
@@ -1412,8 +1430,11 @@ template <uint32_t FEAT>
 __device__ __forceinline__ void trace_body(const TraceParams& P) {
     extern __shared__ int32_t lds_stack[];
     const uint32_t lane = __lane_id();
-    int32_t* stk = lds_stack + threadIdx.x;
-    Tracer<FEAT> T(P, stk, blockDim.x);
+    const size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const LaneStack lane_stack{lds_stack + threadIdx.x, blockDim.x, P.stack_size, P.stack_total,
+                               P.stack_spill ? P.stack_spill + gl * (P.stack_total - P.stack_size)
+                                             : lds_stack + threadIdx.x};
+    Tracer<FEAT> T(P, lane_stack);
     // hot-facet counters after the stacks (stack bytes are a multiple of 8)
     double* hot = reinterpret_cast<double*>(lds_stack + (size_t)blockDim.x * P.stack_size);
     for (uint32_t i = threadIdx.x; i < 8u * P.n_hot; i += blockDim.x) hot[i] = 0.0;
@@ -1631,7 +1652,9 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
     int32_t skip = p.last_hit ? p.last_hit[i] : -1;
     if (skip >= 0) skip += (int32_t)p.facet_base;
-    HitRec h = traverse<false>(tp, p.root, P, D, skip, 1e300, lds_stack + threadIdx.x, blockDim.x, p.stack_size,
+    // the whole stack is in LDS here (cap == lds_cap): the spill pointer is never used
+    const LaneStack S{lds_stack + threadIdx.x, blockDim.x, p.stack_size, p.stack_size, lds_stack + threadIdx.x};
+    HitRec h = traverse<false>(tp, p.root, P, D, skip, 1e300, S,
                                nullptr, 0, nullptr, 0.0, true);
     p.out_facet[i] = h.facet >= 0 ? h.facet - (int32_t)p.facet_base : -1;
     p.out_t[i] = h.facet >= 0 ? h.t : 0.0;
@@ -1674,20 +1697,22 @@ namespace {
 typedef void (*TraceKernel)(TraceParams);
 // [feature set][waves per SIMD - 1]; null where not instantiated
 TraceKernel kernel_for(uint32_t features, int waves) {
-    static TraceKernel lean[4] = {nullptr, k_trace<kLean, 2>, k_trace<kLean, 3>, k_trace<kLean, 4>};
-    static TraceKernel full[4] = {k_trace<kFull, 1>, k_trace<kFull, 2>, k_trace<kFull, 3>, nullptr};
-    static TraceKernel adj[4] = {k_trace<kFullAdj, 1>, k_trace<kFullAdj, 2>, nullptr, nullptr};
-    static TraceKernel lowflux[4] = {nullptr, k_trace<kLowFlux, 2>, k_trace<kLowFlux, 3>, k_trace<kLowFlux, 4>};
-    static TraceKernel lowflux_adj[4] = {nullptr, k_trace<kLowFluxAdj, 2>, k_trace<kLowFluxAdj, 3>,
-                                         k_trace<kLowFluxAdj, 4>};
+    constexpr int N = 5;   // occupancy bounds 1..5 waves per SIMD
+    static TraceKernel lean[N] = {nullptr, k_trace<kLean, 2>, k_trace<kLean, 3>, k_trace<kLean, 4>, k_trace<kLean, 5>};
+    static TraceKernel full[N] = {k_trace<kFull, 1>, k_trace<kFull, 2>, k_trace<kFull, 3>, nullptr, nullptr};
+    static TraceKernel adj[N] = {k_trace<kFullAdj, 1>, k_trace<kFullAdj, 2>, nullptr, nullptr, nullptr};
+    static TraceKernel lowflux[N] = {nullptr, k_trace<kLowFlux, 2>, k_trace<kLowFlux, 3>, k_trace<kLowFlux, 4>,
+                                     k_trace<kLowFlux, 5>};
+    static TraceKernel lowflux_adj[N] = {nullptr, k_trace<kLowFluxAdj, 2>, k_trace<kLowFluxAdj, 3>,
+                                         k_trace<kLowFluxAdj, 4>, nullptr};
     TraceKernel* t = features == kLean ? lean
                    : features == kFull ? full
                    : features == kLowFlux ? lowflux
                    : features == kLowFluxAdj ? lowflux_adj : adj;
     if (waves < 1) waves = 1;
-    if (waves > 4) waves = 4;
+    if (waves > N) waves = N;
     for (int w = waves; w >= 1; --w) if (t[w - 1]) return t[w - 1];
-    for (int w = waves; w <= 4; ++w) if (t[w - 1]) return t[w - 1];
+    for (int w = waves; w <= N; ++w) if (t[w - 1]) return t[w - 1];
     return t[0];
 }
 }  // namespace
@@ -1695,7 +1720,8 @@ TraceKernel kernel_for(uint32_t features, int waves) {
 int mfg_trace_variants(uint32_t features, int* waves_out, int cap) {
     uint32_t f = mfg_trace_kernel_features(features);
     int lo = 1, hi = 2;
-    if (f == kLean || f == kLowFlux || f == kLowFluxAdj) { lo = 2; hi = 4; }
+    if (f == kLean || f == kLowFlux) { lo = 2; hi = 5; }
+    else if (f == kLowFluxAdj) { lo = 2; hi = 4; }
     else if (f == kFull) hi = 3;
     int n = 0;
     for (int w = lo; w <= hi && n < cap; ++w) waves_out[n++] = w;
