@@ -19,7 +19,9 @@
 
 #define MFG_BVH_WIDTH 4
 #define MFG_MAX_STACK 64
+#ifndef MFG_LDS_STACK
 #define MFG_LDS_STACK 16         // traversal stack entries per lane kept in LDS by the tracer
+#endif
 #define MFG_MAX_GROUPS 64
 #define MFG_MAXWELL_GUIDE 128
 #define MFG_MAX_HOT 32           // facets whose counters are privatised per block
