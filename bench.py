@@ -178,7 +178,7 @@ def main():
         # its probe runs: it zeroes the tallies afterwards)
         tr = sim.autotune(args.des_per_step)
         tune = {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu,
-                "probe_hits_per_s": tr.hits_per_s}
+                "bvh_width": tr.bvh_width, "hot_facets": tr.n_hot, "probe_hits_per_s": tr.hits_per_s}
     max_events = 50_000_000
     for _ in range(args.warmup):
         sim.run(args.des_per_step, max_events=max_events)

@@ -290,8 +290,9 @@ int mfg_load_model(mfg_context* ctx, const mfg_model_view* model);
 
 /* Build the per-structure BVH. Replaces RebuildAccelStructure /
  * BuildAccelStructure (MolflowSimulation.cpp:75-88, MolflowSimGeom.cpp:124-184).
- * width = children per node (2, 4 or 8; 0 = default 4), leaf_size = max
- * facets per leaf (0 = default 2, the reference's setting). */
+ * width = children per node (4 or 8; 0 = default 8; nodes are one 128 B
+ * line with 8-bit quantised child boxes either way), leaf_size = max facets
+ * per leaf (0 = default 2, the reference's setting). */
 int mfg_build_accel(mfg_context* ctx, int width, int leaf_size);
 
 /* Zero tallies and restart the particle stream for a seed. Replaces
@@ -387,7 +388,8 @@ int mfg_in_flight(const mfg_context* ctx, uint64_t* n);
 int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);
 
 /* Launch autotuner: times short probe runs of n_probe desorptions for every
- * compiled occupancy bound and block size on the loaded model(s), keeps the
+ * BVH width (8, 4), compiled occupancy bound and block size on the loaded
+ * model(s) (particles stay in flight between probes), keeps the
  * fastest, then zeroes the tallies and puts the particle streams back where
  * they were, so the probes leave no trace in later results. */
 typedef struct mfg_tune_result {
@@ -397,6 +399,8 @@ typedef struct mfg_tune_result {
     uint32_t n_hot;          /* facets whose slot-0 counters are summed per block in LDS
                                 (those with >= 0.5 % of the probes' counter updates) */
     double hits_per_s;       /* probe throughput of the choice */
+    uint32_t bvh_width;      /* children per BVH node of the choice (the BVH is rebuilt with it) */
+    uint32_t reserved;
 } mfg_tune_result;
 int mfg_autotune(mfg_context* ctx, uint64_t n_probe, mfg_tune_result* out);
 

@@ -288,6 +288,39 @@ inline float round_up(double x) { float f = (float)x; if ((double)f < x) f = std
 
 // Collapse a binary tree into 4-wide nodes (open the largest inner child
 // until 4 children), boxes padded by `pad` and rounded outwards to FP32.
+// Quantised child bounds of one axis: the FP32 values the kernel computes,
+// o + q * 2^e (one rounding), must enclose [lo, hi].
+struct AxisQuant {
+    float o;
+    int e;
+    static float deq(float o, int e, int q) { return o + (float)q * std::ldexp(1.0f, e); }
+    // smallest step 2^e with which every child box fits in 0..255 steps
+    static AxisQuant make(double node_lo, double node_hi) {
+        AxisQuant a;
+        a.o = round_down(node_lo);
+        double ext = std::max(node_hi - (double)a.o, 1e-30);
+        a.e = (int)std::ceil(std::log2(ext / 250.0));
+        a.e = std::max(-126, std::min(a.e, 120));
+        while (a.e < 120 && (double)deq(a.o, a.e, 255) < node_hi) ++a.e;
+        return a;
+    }
+    uint8_t qlo(double lo) const {
+        int q = (int)std::floor(((double)lo - (double)o) / std::ldexp(1.0, e));
+        q = std::max(0, std::min(255, q));
+        while (q > 0 && (double)deq(o, e, q) > lo) --q;
+        return (uint8_t)q;
+    }
+    uint8_t qhi(double hi) const {
+        int q = (int)std::ceil(((double)hi - (double)o) / std::ldexp(1.0, e));
+        q = std::max(0, std::min(255, q));
+        while (q < 255 && (double)deq(o, e, q) < hi) ++q;
+        if ((double)deq(o, e, q) < hi) throw Err(MFG_ERR_INVALID, "BVH box quantisation failed");
+        return (uint8_t)q;
+    }
+};
+
+// Collapses the binary tree into nodes of up to `width` children (4 or 8):
+// the child with the largest surface is opened until the node is full.
 struct Collapse {
     const std::vector<B2>& b2;
     const std::vector<int>& prims;
@@ -295,14 +328,15 @@ struct Collapse {
     std::vector<int32_t>& leaf_ids;
     int32_t node_base, leaf_base;
     double pad;
+    int width = MFG_BVH_WIDTH;
     int max_depth = 0;
     int build(int n2, int depth) {
         max_depth = std::max(max_depth, depth);
-        int ch[4]; int nc = 0;
+        int ch[8]; int nc = 0;
         const B2& root = b2[n2];
         if (root.left < 0) { ch[nc++] = n2; }
         else { ch[nc++] = root.left; ch[nc++] = root.right; }
-        while (nc < 4) {
+        while (nc < width) {
             int best = -1; double ba = -1;
             for (int i = 0; i < nc; ++i) if (b2[ch[i]].left >= 0 && area(b2[ch[i]].b) > ba) { ba = area(b2[ch[i]].b); best = i; }
             if (best < 0) break;
@@ -314,17 +348,24 @@ struct Collapse {
         out.push_back(BvhNode());
         BvhNode nd;
         std::memset(&nd, 0, sizeof(nd));
-        for (int c = 0; c < 4; ++c) {
-            if (c >= nc) {
-                nd.lox[c] = nd.loy[c] = nd.loz[c] = INFINITY;
-                nd.hix[c] = nd.hiy[c] = nd.hiz[c] = -INFINITY;
+        Box all = empty_box();
+        for (int c = 0; c < nc; ++c) grow(all, b2[ch[c]].b);
+        AxisQuant q[3];
+        for (int k = 0; k < 3; ++k) q[k] = AxisQuant::make(all.lo[k] - pad, all.hi[k] + pad);
+        nd.ox = q[0].o; nd.oy = q[1].o; nd.oz = q[2].o;
+        nd.ex = (int8_t)q[0].e; nd.ey = (int8_t)q[1].e; nd.ez = (int8_t)q[2].e;
+        for (int c = 0; c < 8; ++c) {
+            if (c >= nc) {   // empty slot: an inverted box, never hit
+                for (int k = 0; k < 3; ++k) { nd.qlo[k][c] = 255; nd.qhi[k][c] = 0; }
                 nd.child[c] = 0; nd.count[c] = 0;
                 continue;
             }
             const B2& cn = b2[ch[c]];
-            nd.valid |= 1u << c;
-            nd.lox[c] = round_down(cn.b.lo[0] - pad); nd.loy[c] = round_down(cn.b.lo[1] - pad); nd.loz[c] = round_down(cn.b.lo[2] - pad);
-            nd.hix[c] = round_up(cn.b.hi[0] + pad); nd.hiy[c] = round_up(cn.b.hi[1] + pad); nd.hiz[c] = round_up(cn.b.hi[2] + pad);
+            nd.valid |= (uint8_t)(1u << c);
+            for (int k = 0; k < 3; ++k) {
+                nd.qlo[k][c] = q[k].qlo(cn.b.lo[k] - pad);
+                nd.qhi[k][c] = q[k].qhi(cn.b.hi[k] + pad);
+            }
             if (cn.left < 0) {
                 if (cn.count > 255) throw Err(MFG_ERR_LIMIT, "BVH leaf with more than 255 facets");
                 int32_t first = (int32_t)leaf_ids.size() + leaf_base;
@@ -726,7 +767,7 @@ struct mfg_context {
     bool loaded = false, accel = false;
     uint32_t stack_size = 0;             // traversal stack entries a walk may need
     DevBuf<int32_t> stack_spill;         // per-lane entries beyond the LDS part
-    int bvh_width = 4, leaf_size = 2;
+    int bvh_width = MFG_BVH_WIDTH, leaf_size = 2;   // children per BVH node, facets per leaf
     // device data
     DevBuf<DFacet> facets;
     DevBuf<DPhys> phys;
@@ -998,7 +1039,9 @@ void zero_tallies(mfg_context* c) {
 
 void build_accel(mfg_context* c, int width, int leaf) {
     if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
-    if (width != 0 && width != 4) throw Err(MFG_ERR_INVALID, "this build packs 4-wide BVH nodes (width 4)");
+    if (width == 0) width = MFG_BVH_WIDTH;
+    if (width != 4 && width != 8) throw Err(MFG_ERR_INVALID, "BVH width must be 4 or 8 (0 = 8)");
+    c->bvh_width = width;
     if (leaf <= 0) leaf = 2;
     if (leaf > 255) throw Err(MFG_ERR_INVALID, "leaf size above 255");
     c->leaf_size = leaf;
@@ -1022,7 +1065,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
                     build_b2(b2, prims, M.boxes, leaf);
                     for (auto& p : prims) p += (int)c->dvars[vi].facet_base;
                     // indices are variant-local here, made global after all variants are built
-                    Collapse col{b2, prims, vnodes[vi], vleaf[vi], 0, 0, pad};
+                    Collapse col{b2, prims, vnodes[vi], vleaf[vi], 0, 0, pad, width};
                     int r = col.build(0, 1);
                     vroots[vi].push_back(r);
                     vdepth[vi] = std::max(vdepth[vi], col.max_depth);
@@ -1046,7 +1089,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
     for (size_t vi = 0; vi < nv; ++vi) {
         int32_t nb = (int32_t)nodes.size(), lb = (int32_t)leafs.size();
         for (BvhNode nd : vnodes[vi]) {
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < 8; ++k) {
                 if (!(nd.valid & (1u << k))) continue;
                 if (nd.count[k] > 0) nd.child[k] = ~((~nd.child[k]) + lb);
                 else nd.child[k] += nb;
@@ -1058,8 +1101,8 @@ void build_accel(mfg_context* c, int width, int leaf) {
         for (int32_t r : vroots[vi]) roots.push_back(r < 0 ? -1 : r + nb);
         depth = std::max(depth, vdepth[vi]);
     }
-    // stack entries: a walk pushes at most 4 per level
-    uint32_t need = (uint32_t)(4 * depth + 4);
+    // stack entries: a walk pushes at most width-1 per level
+    uint32_t need = (uint32_t)((width - 1) * depth + width);
     if (need > MFG_MAX_STACK) throw Err(MFG_ERR_LIMIT, "BVH too deep for the LDS traversal stack (depth " + std::to_string(depth) + ")");
     c->stack_size = need;
     hipStream_t s = c->stream;
@@ -1629,24 +1672,36 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
         size_t nv = c->dvars.size();
         std::vector<uint64_t> nd(nv, std::max<uint64_t>(1, n_probe / nv)), seeds(nv, c->seed);
         std::vector<uint64_t> save_pid = c->pid_base;
+        const bool save_carry = c->carry_on;
+        // probes keep particles in flight between launches (when the adjoint
+        // allows it), so each timed probe runs at the steady-state rate
+        // rather than waiting for its longest trajectories
+        c->carry_on = c->adj_groups == 0;
         double best = -1;
         mfg_tune_result r{};
-        // one untimed probe loads the code objects
-        c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
-        run_variants(c, nd.data(), seeds.data(), 0, nullptr);
-        for (int i = 0; i < nw; ++i)
-            for (uint32_t b : blocks) {
-                c->waves = wl[i]; c->block = b; c->blocks_per_cu = 0;
-                mfg_step_result sr{};
-                run_variants(c, nd.data(), seeds.data(), 0, &sr);
-                double hs = sr.kernel_ms > 0 ? (double)sr.hits / (sr.kernel_ms * 1e-3) : 0.0;
-                if (hs > best) {
-                    best = hs;
-                    int grid, block, waves;
-                    launch_shape(c, feats, grid, block, waves);
-                    r.waves = (uint32_t)waves; r.block = b; r.blocks_per_cu = (uint32_t)(grid / c->n_cu); r.hits_per_s = hs;
+        const int widths[] = {8, 4};
+        for (int width : widths) {
+            if (width != c->bvh_width) build_accel(c, width, c->leaf_size);
+            // one untimed probe per width loads the code objects and fills the lanes
+            c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
+            run_variants(c, nd.data(), seeds.data(), 0, nullptr);
+            for (int i = 0; i < nw; ++i)
+                for (uint32_t b : blocks) {
+                    c->waves = wl[i]; c->block = b; c->blocks_per_cu = 0;
+                    mfg_step_result sr{};
+                    run_variants(c, nd.data(), seeds.data(), 0, &sr);
+                    double hs = sr.kernel_ms > 0 ? (double)sr.hits / (sr.kernel_ms * 1e-3) : 0.0;
+                    if (hs > best) {
+                        best = hs;
+                        int grid, block, waves;
+                        launch_shape(c, feats, grid, block, waves);
+                        r.waves = (uint32_t)waves; r.block = b; r.blocks_per_cu = (uint32_t)(grid / c->n_cu);
+                        r.hits_per_s = hs; r.bvh_width = (uint32_t)width;
+                    }
                 }
-            }
+        }
+        if ((int)r.bvh_width != c->bvh_width) build_accel(c, (int)r.bvh_width, c->leaf_size);
+        c->carry_on = save_carry;
         c->waves = (int)r.waves; c->block = r.block; c->blocks_per_cu = 0;
         // hot facets: those that took at least 1/2 % of the probes' slot-0
         // counter updates (hits + desorptions), most frequent first

@@ -5,7 +5,7 @@
 //   DFacet[nF_total]      128 B, one cache line per ray/facet test
 //   DPhys[nF_total]       physics and recording parameters, read on a hit
 //   double2 poly[]        normalised (u,v) polygon vertices
-//   BvhNode[]             4-wide nodes, 128 B each, FP32 boxes
+//   BvhNode[]             up to 8-wide nodes, 128 B each, 8-bit quantised boxes
 //   LeafFacet[]           the ray/facet test data in leaf order (128 B) and
 //   LeafUV[]              triangle (u,v) polygons (64 B): a leaf is a
 //                         contiguous run, no index indirection
@@ -17,7 +17,7 @@
 
 #include "../../include/molflow_gpu.h"
 
-#define MFG_BVH_WIDTH 4
+#define MFG_BVH_WIDTH 8          // children per node (a 4-wide build uses 4 of the 8 slots)
 #define MFG_MAX_STACK 64
 #ifndef MFG_LDS_STACK
 #define MFG_LDS_STACK 16         // traversal stack entries per lane kept in LDS by the tracer
@@ -117,12 +117,19 @@ struct alignas(16) DPhys {
     int32_t pad2_;
 };
 
-struct alignas(16) BvhNode {
-    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
-    int32_t child[4];   // >= 0: node index; < 0: leaf, first facet slot = ~child
-    uint8_t count[4];   // facets in a leaf child (0 for inner/empty)
-    uint32_t valid;     // bit c set when child slot c is used
-    uint32_t pad[2];
+// One BVH node of up to 8 children in one 128 B line. Child boxes are
+// quantised to 8 bits per plane relative to the node's origin with a
+// power-of-two step per axis: lo = o + qlo * 2^e, hi = o + qhi * 2^e in FP32
+// (the builder checks that these FP32 values enclose the padded FP64 box).
+struct alignas(128) BvhNode {
+    float ox, oy, oz;           // quantisation origin
+    int8_t ex, ey, ez;          // per-axis step exponents
+    uint8_t valid;              // bit c set when child slot c is used
+    int32_t child[8];           // >= 0: node index; < 0: leaf, first leaf record = ~child
+    uint8_t count[8];           // facets in a leaf child (0 for inner/empty)
+    uint8_t qlo[3][8];          // [axis][child]
+    uint8_t qhi[3][8];
+    uint8_t pad[24];
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 bytes");
 

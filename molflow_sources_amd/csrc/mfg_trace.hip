@@ -257,44 +257,59 @@ __device__ __forceinline__ Ray32 make_ray32(d3 P, d3 D) {
     return {(float)P.x, (float)P.y, (float)P.z, safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
 }
 
-// Slab test of a node's 4 children against [0, tmax]: hit mask + entry t.
-// Boxes are padded at build time, so the FP32 test never drops a box that
-// holds an FP64 hit.
-// A node's 128 B as eight 16-byte loads issued together (one round trip).
+// A node's 128 B (112 used) as seven 16-byte loads issued together (one
+// round trip). Words: c0 = origin x,y,z + (ex, ey, ez, valid) bytes;
+// c1, c2 = child words 0-3, 4-7; c3 = leaf counts (x,y) + qlo x (z,w);
+// c4 = qlo y, qlo z; c5 = qhi x, qhi y; c6 = qhi z (x,y).
 struct NodeRegs {
-    float4 lx, ly, lz, hx, hy, hz;
-    int4 ch;        // child words
-    uint4 meta;     // x: the four leaf counts (byte c = slot c), y: valid mask
+    uint4 c0, c1, c2, c3, c4, c5, c6;
 };
-static_assert(sizeof(NodeRegs) == sizeof(BvhNode), "NodeRegs mirrors BvhNode");
 
 __device__ __forceinline__ NodeRegs load_node(const BvhNode* __restrict__ nd) {
-    const float4* q = reinterpret_cast<const float4*>(nd);
+    const uint4* q = reinterpret_cast<const uint4*>(nd);
     NodeRegs n;
-    n.lx = q[0]; n.ly = q[1]; n.lz = q[2]; n.hx = q[3]; n.hy = q[4]; n.hz = q[5];
-    n.ch = reinterpret_cast<const int4*>(nd)[6];
-    n.meta = reinterpret_cast<const uint4*>(nd)[7];
+    n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6];
     return n;
 }
 
-__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const Ray32& r,
-                                     float tmax, float& tn) {
-    float tx0 = (lx - r.ox) * r.ix, tx1 = (hx - r.ox) * r.ix;
-    float ty0 = (ly - r.oy) * r.iy, ty1 = (hy - r.oy) * r.iy;
-    float tz0 = (lz - r.oz) * r.iz, tz1 = (hz - r.oz) * r.iz;
-    float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-    float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    tn = a;
-    return a <= b * 1.000001f;
+__device__ __forceinline__ uint32_t node_valid(const NodeRegs& n) { return n.c0.w >> 24; }
+__device__ __forceinline__ int32_t node_child(const NodeRegs& n, uint32_t c) {
+    uint32_t w = c == 0 ? n.c1.x : c == 1 ? n.c1.y : c == 2 ? n.c1.z : c == 3 ? n.c1.w
+               : c == 4 ? n.c2.x : c == 5 ? n.c2.y : c == 6 ? n.c2.z : n.c2.w;
+    return (int32_t)w;
 }
+__device__ __forceinline__ uint32_t node_count(const NodeRegs& n, uint32_t c) {
+    return ((c < 4 ? n.c3.x : n.c3.y) >> (8u * (c & 3u))) & 0xffu;
+}
+// 2^e as a float (|e| < 127)
+__device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
-__device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[4]) {
+// Slab test of the node's children against [0, tmax]: hit mask + entry t.
+// The quantised boxes enclose the padded FP64 boxes, so the FP32 test never
+// drops a box that holds an FP64 hit.
+__device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[8]) {
+    const float ox = __uint_as_float(n.c0.x), oy = __uint_as_float(n.c0.y), oz = __uint_as_float(n.c0.z);
+    const uint32_t ew = n.c0.w;
+    const float sx = exp2i((int)(int8_t)(ew & 0xffu)), sy = exp2i((int)(int8_t)((ew >> 8) & 0xffu));
+    const float sz = exp2i((int)(int8_t)((ew >> 16) & 0xffu));
     uint32_t mask = 0;
-    if (slab(n.lx.x, n.ly.x, n.lz.x, n.hx.x, n.hy.x, n.hz.x, r, tmax, tn[0])) mask |= 1u;
-    if (slab(n.lx.y, n.ly.y, n.lz.y, n.hx.y, n.hy.y, n.hz.y, r, tmax, tn[1])) mask |= 2u;
-    if (slab(n.lx.z, n.ly.z, n.lz.z, n.hx.z, n.hy.z, n.hz.z, r, tmax, tn[2])) mask |= 4u;
-    if (slab(n.lx.w, n.ly.w, n.lz.w, n.hx.w, n.hy.w, n.hz.w, r, tmax, tn[3])) mask |= 8u;
-    return mask & n.meta.y;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) {
+        const uint32_t sh = 8u * (c & 3u);
+        const uint32_t lxw = c < 4 ? n.c3.z : n.c3.w, lyw = c < 4 ? n.c4.x : n.c4.y, lzw = c < 4 ? n.c4.z : n.c4.w;
+        const uint32_t hxw = c < 4 ? n.c5.x : n.c5.y, hyw = c < 4 ? n.c5.z : n.c5.w, hzw = c < 4 ? n.c6.x : n.c6.y;
+        const float lx = fmaf((float)((lxw >> sh) & 0xffu), sx, ox), hx = fmaf((float)((hxw >> sh) & 0xffu), sx, ox);
+        const float ly = fmaf((float)((lyw >> sh) & 0xffu), sy, oy), hy = fmaf((float)((hyw >> sh) & 0xffu), sy, oy);
+        const float lz = fmaf((float)((lzw >> sh) & 0xffu), sz, oz), hz = fmaf((float)((hzw >> sh) & 0xffu), sz, oz);
+        float tx0 = (lx - r.ox) * r.ix, tx1 = (hx - r.ox) * r.ix;
+        float ty0 = (ly - r.oy) * r.iy, ty1 = (hy - r.oy) * r.iy;
+        float tz0 = (lz - r.oz) * r.iz, tz1 = (hz - r.oz) * r.iz;
+        float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        tn[c] = a;
+        if (a <= b * 1.000001f) mask |= 1u << c;
+    }
+    return mask & node_valid(n);
 }
 
 __device__ __forceinline__ float widen(double t) {
@@ -495,19 +510,20 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     int32_t node = root;
     for (;;) {
         const NodeRegs nd = load_node(P.nodes + node);
-        float tn[4];
+        float tn[8];
         uint32_t m = node_hits(nd, r, tbf, tn);
-        const int32_t ch0 = nd.ch.x, ch1 = nd.ch.y, ch2 = nd.ch.z, ch3 = nd.ch.w;
-        const uint32_t cnts = nd.meta.x;
         // leaf children first (one loop for all slots, so lanes whose leaves
-        // sit in different slots test their facets together)
-        uint32_t lm = ((m & 1u) && ch0 < 0 ? 1u : 0u) | ((m & 2u) && ch1 < 0 ? 2u : 0u) |
-                      ((m & 4u) && ch2 < 0 ? 4u : 0u) | ((m & 8u) && ch3 < 0 ? 8u : 0u);
+        // sit in different child slots test their facets together)
+        uint32_t lm = 0, im = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (!(m & (1u << c))) continue;
+            if (node_child(nd, c) < 0) lm |= 1u << c; else im |= 1u << c;
+        }
         while (lm) {
             uint32_t c = (uint32_t)__builtin_ctz(lm);
             lm &= lm - 1u;
-            int32_t ch = c == 0 ? ch0 : (c == 1 ? ch1 : (c == 2 ? ch2 : ch3));
-            uint32_t first = (uint32_t)(~ch), cnt = (cnts >> (8u * c)) & 0xffu;
+            uint32_t first = (uint32_t)(~node_child(nd, c)), cnt = node_count(nd, c);
             for (uint32_t k = 0; k < cnt; ++k) {
                 double t, u, v;
                 int32_t f;
@@ -531,26 +547,29 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         // inner children that survive the (possibly shrunk) tmax, sorted
         // near-to-far with a fixed network (registers only)
         constexpr float INF = 3.4e38f;
-        const bool i0 = (m & 1u) && ch0 >= 0, i1 = (m & 2u) && ch1 >= 0, i2 = (m & 4u) && ch2 >= 0, i3 = (m & 8u) && ch3 >= 0;
-        float k0 = i0 ? tn[0] : INF, k1 = i1 ? tn[1] : INF, k2 = i2 ? tn[2] : INF, k3 = i3 ? tn[3] : INF;
-        int32_t c0 = i0 ? ch0 : -1, c1 = i1 ? ch1 : -1, c2 = i2 ? ch2 : -1, c3 = i3 ? ch3 : -1;
-        if (k0 > tbf) { k0 = INF; c0 = -1; }
-        if (k1 > tbf) { k1 = INF; c1 = -1; }
-        if (k2 > tbf) { k2 = INF; c2 = -1; }
-        if (k3 > tbf) { k3 = INF; c3 = -1; }
-#define MFG_CSWAP(ka, ca, kb, cb) \
-    if (kb < ka) { float tk = ka; ka = kb; kb = tk; int32_t tc = ca; ca = cb; cb = tc; }
-        MFG_CSWAP(k0, c0, k1, c1)
-        MFG_CSWAP(k2, c2, k3, c3)
-        MFG_CSWAP(k0, c0, k2, c2)
-        MFG_CSWAP(k1, c1, k3, c3)
-        MFG_CSWAP(k1, c1, k2, c2)
+        float k[8];
+        int32_t cid[8];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const bool in = (im & (1u << c)) && !(tn[c] > tbf);
+            k[c] = in ? tn[c] : INF;
+            cid[c] = in ? node_child(nd, c) : -1;
+        }
+#define MFG_CSWAP(a, b) \
+    if (k[b] < k[a]) { float tk = k[a]; k[a] = k[b]; k[b] = tk; int32_t tc = cid[a]; cid[a] = cid[b]; cid[b] = tc; }
+        // Batcher odd-even merge sort of 8 keys (19 compare-exchanges)
+        MFG_CSWAP(0, 1) MFG_CSWAP(2, 3) MFG_CSWAP(4, 5) MFG_CSWAP(6, 7)
+        MFG_CSWAP(0, 2) MFG_CSWAP(1, 3) MFG_CSWAP(4, 6) MFG_CSWAP(5, 7)
+        MFG_CSWAP(1, 2) MFG_CSWAP(5, 6)
+        MFG_CSWAP(0, 4) MFG_CSWAP(1, 5) MFG_CSWAP(2, 6) MFG_CSWAP(3, 7)
+        MFG_CSWAP(2, 4) MFG_CSWAP(3, 5)
+        MFG_CSWAP(1, 2) MFG_CSWAP(3, 4) MFG_CSWAP(5, 6)
 #undef MFG_CSWAP
-        if (c0 >= 0) {
-            if (c3 >= 0) S.push(sp, c3);
-            if (c2 >= 0) S.push(sp, c2);
-            if (c1 >= 0) S.push(sp, c1);
-            node = c0;
+        if (cid[0] >= 0) {
+#pragma unroll
+            for (int c = 7; c >= 1; --c)
+                if (cid[c] >= 0) S.push(sp, cid[c]);
+            node = cid[0];
             continue;
         }
         if (sp == 0) break;
@@ -1343,17 +1362,17 @@ struct Tracer {
                 double save_u = hu, save_v = hv;
                 for (;;) {
                     const NodeRegs nd = load_node(P.nodes + node);
-                    float tn[4];
+                    float tn[8];
                     uint32_t msk = node_hits(nd, r, tbf, tn);
                     while (msk) {
                         uint32_t c = (uint32_t)__builtin_ctz(msk);
                         msk &= msk - 1u;
-                        int32_t ch = c == 0 ? nd.ch.x : (c == 1 ? nd.ch.y : (c == 2 ? nd.ch.z : nd.ch.w));
+                        int32_t ch = node_child(nd, c);
                         if (ch >= 0) {
                             S.push(sp, ch);
                             continue;
                         }
-                        uint32_t first = (uint32_t)(~ch), cnt = (nd.meta.x >> (8u * c)) & 0xffu;
+                        uint32_t first = (uint32_t)(~ch), cnt = node_count(nd, c);
                         for (uint32_t kk = 0; kk < cnt; ++kk) {
                             double t, u, v;
                             int32_t f;
