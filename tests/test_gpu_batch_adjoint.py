@@ -219,3 +219,50 @@ def test_adjoint_matches_central_differences(sim):
         fd = (up @ forward(sp)[0] - up @ forward(sm)[0]) / (2 * ds)
         # common random numbers: the paths are the same, only weights change
         assert abs(fd - grad[gi]) <= 2e-3 * max(abs(grad[gi]), 1e-3 * np.abs(grad).max()) + 1e-9, (gi, fd, grad[gi])
+
+
+def test_bvh_width_and_autotune_do_not_change_results(sim):
+    """The traversal order (BVH width 4 or 8), the kernel variant, the launch
+    shape and the hot-facet privatisation chosen by the autotuner change no
+    particle's events: integer tallies are identical for one seed."""
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=4, n_anglemaps=2)
+    sim.load(m)
+    res = []
+    for width in (8, 4):
+        sim.rebuild_accel(width, 2)
+        sim.reset(seed=12)
+        sim.run(20_000, max_events=MAXEV)
+        t = sim.read()
+        res.append(int_tallies(t) + (t.angle_maps.copy(),))
+    t = sim.autotune(20_000)
+    assert t.bvh_width in (4, 8) and t.hits_per_s > 0
+    sim.reset(seed=12)
+    sim.run(20_000, max_events=MAXEV)
+    tt = sim.read()
+    res.append(int_tallies(tt) + (tt.angle_maps.copy(),))
+    sim.set_kernel_variant(0)
+    sim.set_launch(256, 0)
+    for other in res[1:]:
+        assert res[0][0] == other[0] and res[0][1] == other[1]
+        assert np.array_equal(res[0][2], other[2]) and np.array_equal(res[0][3], other[3])
+        assert np.array_equal(res[0][4], other[4])
+
+
+def test_hot_facets_from_autotune_keep_tube_tallies(sim):
+    m = geometry.tube_model()
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=4)
+    sim.run(200_000, max_events=MAXEV)
+    a = sim.read()
+    t = sim.autotune(100_000)
+    assert t.n_hot >= 2          # inlet and outlet take every desorption and absorption
+    sim.reset(seed=4)
+    sim.run(200_000, max_events=MAXEV)
+    b = sim.read()
+    sim.set_kernel_variant(0)
+    sim.set_launch(256, 0)
+    assert np.array_equal(a.facets["nb_mc_hit"], b.facets["nb_mc_hit"])
+    assert np.array_equal(a.facets["nb_desorbed"], b.facets["nb_desorbed"])
+    for f in ("nb_abs_equiv", "sum_1_per_ort_velocity", "sum_v_ort", "sum_1_per_velocity"):
+        assert np.allclose(a.facets[f], b.facets[f], rtol=1e-12, atol=0)

@@ -61,6 +61,21 @@ def test_first_hit_bit_exact(gpu, name, make, box):
     assert np.allclose(ug[hit], uc[hit], atol=1e-9) and np.allclose(vg[hit], vc[hit], atol=1e-9)
 
 
+def test_first_hit_bit_exact_4_wide(gpu):
+    """The same fixed ray batch through a 4-wide build of the same format."""
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+    lo, hi = m.vertices.min(0), m.vertices.max(0)
+    o, d = random_rays(50_000, lo - 0.05 * (hi - lo), hi + 0.05 * (hi - lo), seed=17)
+    gpu.load(m)
+    gpu.rebuild_accel(4, 2)
+    fg, tg, _, _ = gpu.intersect(o, d)
+    fc, tc, _, _ = CpuModel(m).intersect(o, d, brute=True)
+    gpu.rebuild_accel()
+    assert np.array_equal(fg, fc), f"{int((fg != fc).sum())} first-hit facets differ"
+    hit = fc >= 0
+    assert np.all(np.abs(tg[hit] - tc[hit]) <= 1e-5 * tc[hit])
+
+
 def test_first_hit_every_structure(gpu):
     m, _ = model_from_fixture("05-three_structures_nonsquare_textures")
     loaded(gpu, m)
