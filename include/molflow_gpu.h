@@ -266,7 +266,8 @@ typedef struct mfg_tally_view {
 /* Result of one mfg_run (MCStepResult, Particle.h:24-28, plus counts). */
 typedef struct mfg_step_result {
     int32_t status;               /* 0 Success, 1 DesorptionError, 2 MaxReached */
-    int32_t stopped;              /* 1 if mfg_request_stop ended the run early */
+    int32_t stopped;              /* 1 if mfg_request_stop ended the run early,
+                                     2 if the launch time limit did */
     uint64_t desorbed;            /* particles started by this call */
     uint64_t hits;                /* nbMCHit added by this call */
     double kernel_ms;             /* device time of the trace kernel(s) */
@@ -309,6 +310,15 @@ int mfg_run(mfg_context* ctx, uint64_t n_des, uint64_t max_events, mfg_step_resu
 /* Ask a running mfg_run (from another thread) to stop desorbing; in-flight
  * particles finish. The exitRequested analogue (Particle.h:122). */
 int mfg_request_stop(mfg_context* ctx);
+
+/* Wall-clock limit of one trace launch (default 60 s, 0 < seconds <= 86400).
+ * A wave that reaches it starts no particle, parks its particles in flight
+ * (carry enabled) or ends them where they are (carry disabled), and the run
+ * returns with stopped = 2. It bounds every launch, also one whose particles
+ * can never be absorbed and whose max_events is 0. No counterpart in the
+ * reference, whose SimulationMCStep returns after a given number of events
+ * (Particle.cpp:207-396) instead. */
+int mfg_set_time_limit(mfg_context* ctx, double seconds);
 
 /* Sizes and per-facet offsets of the tally arrays. kind: 0 texture cell
  * offset, 1 direction cell offset, 2 profile slice offset, 3 angle-map cell

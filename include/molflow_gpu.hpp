@@ -79,6 +79,8 @@ public:
 
     /* exitRequested (Particle.h:122) */
     void RequestStop() { check(mfg_request_stop(ctx_), "mfg_request_stop"); }
+    /* Wall-clock limit of one trace launch (a run it ends reports stopped = 2) */
+    void SetTimeLimit(double seconds) { check(mfg_set_time_limit(ctx_, seconds), "mfg_set_time_limit"); }
 
     /* UpdateMCHits (Particle.cpp:21-73): adds the device tallies into the
      * caller's buffers (laid out as GlobalSimuState, see molflow_gpu.h) */

@@ -138,6 +138,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mfg_reset": (C.c_int, [vp, u64, u64]),
         "mfg_run": (C.c_int, [vp, u64, u64, C.POINTER(StepResult)]),
         "mfg_request_stop": (C.c_int, [vp]),
+        "mfg_set_time_limit": (C.c_int, [vp, C.c_double]),
         "mfg_get_layout": (C.c_int, [vp, C.POINTER(TallyLayout)]),
         "mfg_get_facet_offsets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), C.c_size_t]),
         "mfg_read_tallies": (C.c_int, [vp, C.POINTER(TallyView)]),
@@ -173,7 +174,7 @@ def load_library(path: str | None = None) -> C.CDLL:
 
 EXPORTED_SYMBOLS = (
     "mfg_create", "mfg_destroy", "mfg_last_error", "mfg_abi_version", "mfg_load_model", "mfg_build_accel",
-    "mfg_reset", "mfg_run", "mfg_request_stop", "mfg_get_layout", "mfg_get_facet_offsets", "mfg_read_tallies",
+    "mfg_reset", "mfg_run", "mfg_request_stop", "mfg_set_time_limit", "mfg_get_layout", "mfg_get_facet_offsets", "mfg_read_tallies",
     "mfg_hits_size", "mfg_intersect_batch", "mfg_maxwell_cdf", "mfg_parse_user_moment",
     "mfg_integrated_desorption", "mfg_set_trace", "mfg_get_trace", "mfg_load_variants", "mfg_run_variants",
     "mfg_get_variant_layout", "mfg_read_variant_tallies", "mfg_set_launch", "mfg_enable_graph",

@@ -791,13 +791,15 @@ struct mfg_context {
     DevBuf<mfg_texture_cell> tex;
     DevBuf<mfg_direction_cell> dir;
     DevBuf<mfg_profile_slice> prof;
-    DevBuf<unsigned long long> am, counters;   // counters: next_particle, log_count, leak_count, -, carry control[3]
+    DevBuf<unsigned long long> am, counters;   // counters: next_particle, log_count, leak_count, time-limit flag, carry control[3]
     DevBuf<DGlobal> glob;
     DevBuf<mfg_log_entry> log;
     DevBuf<double> leaks;
     uint32_t log_cap = 0;
     // host-mapped stop flag
     int32_t* stop_host = nullptr;
+    double time_limit_s = 60.0;          // wall-clock limit of one trace launch
+    uint64_t clock_khz = 0;              // device wall-clock rate
     int32_t* stop_dev = nullptr;
     // pinned run block (variants + prefix), read by graph replays
     DVariant* pin_vars = nullptr;
@@ -1118,8 +1120,8 @@ void build_accel(mfg_context* c, int width, int leaf) {
         for (int k = 0; k < 3; ++k) { L.O[k] = F.O[k]; L.N[k] = F.N[k]; L.U[k] = F.U[k]; L.V[k] = F.V[k]; }
         L.invU2 = F.invU2; L.invV2 = F.invV2;
         L.id = leafs[i]; L.flags = F.flags; L.poly_off = F.poly_off; L.poly_n = F.poly_n;
-        if (F.poly_n == 3)
-            for (int k = 0; k < 6; ++k) luv[i].uv[k] = c->h_poly[2 * (size_t)F.poly_off + k];
+        if (F.poly_n == 3 || F.poly_n == 4)
+            for (uint32_t k = 0; k < 2 * F.poly_n; ++k) luv[i].uv[k] = c->h_poly[2 * (size_t)F.poly_off + k];
     }
     c->leaves.upload(lf, s);
     c->leaf_uv.upload(luv, s);
@@ -1149,6 +1151,8 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.total = total;
     p.max_events = max_events;
     p.stop_flag = c->stop_dev;
+    p.time_limit = (uint64_t)(c->time_limit_s * (double)c->clock_khz * 1000.0);
+    p.expired = c->counters.p + 3;
     p.carry = c->carry.p; p.carry_ctl = c->counters.p + 4; p.carry_cap = c->carry_cap;
     p.carry_keep = c->carry_on ? 1u : 0u;
     p.trace_facet = c->trace_f.p; p.trace_kind = c->trace_kind.p; p.trace_n = c->trace_n; p.trace_k = c->trace_k;
@@ -1235,13 +1239,15 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
     key.push_back((uint64_t)waves);
     key.push_back((uint64_t)(uintptr_t)p.carry); key.push_back(p.carry_keep);
-    key.push_back((uint64_t)(uintptr_t)p.hot_fc); key.push_back(p.n_hot);
+    key.push_back((uint64_t)(uintptr_t)p.hot_fc); key.push_back(p.n_hot); key.push_back(p.time_limit);
     // the events bracket the trace kernel alone (eager path) or the whole
     // graph (replay path)
     auto enqueue = [&](bool events) {
         HIPCHK(hipMemcpyAsync(c->variants.p, c->pin_vars, nv * sizeof(DVariant), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(c->prefix.p, c->pin_prefix, (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-        if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s)) throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
+        if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s) ||
+            mfg_launch_zero(c->counters.p + 3, sizeof(unsigned long long), s))
+            throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
         if (events) HIPCHK(hipEventRecord(c->ev0, s));
         if (mfg_launch_trace(p, feats, waves, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
         if (p.carry && mfg_launch_carry_flip(p.carry_ctl, s)) throw Err(MFG_ERR_DEVICE, "carry flip launch failed");
@@ -1266,6 +1272,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     std::vector<DGlobal> after(nv);
     HIPCHK(hipMemcpy(after.data(), c->glob.p, nv * sizeof(DGlobal), hipMemcpyDeviceToHost));
+    unsigned long long expired = 0;
+    HIPCHK(hipMemcpy(&expired, c->counters.p + 3, sizeof(expired), hipMemcpyDeviceToHost));
     if (p.carry) {
         unsigned long long ctl[3];
         HIPCHK(hipMemcpy(ctl, c->counters.p + 4, sizeof(ctl), hipMemcpyDeviceToHost));
@@ -1280,7 +1288,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     if (c->adj_groups) c->adj_des += des;
     if (res) {
         res->status = 0;
-        res->stopped = *c->stop_host ? 1 : 0;
+        res->stopped = expired ? 2 : *c->stop_host ? 1 : 0;
         res->desorbed = des;
         res->hits = hits;
         res->kernel_ms = ms;
@@ -1436,6 +1444,10 @@ int mfg_create(mfg_context** out, int device) {
     *c->stop_host = 0;
     if (hipHostGetDevicePointer((void**)&c->stop_dev, c->stop_host, 0) != hipSuccess) return MFG_ERR_DEVICE;
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return MFG_ERR_DEVICE;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+        return MFG_ERR_DEVICE;
+    c->clock_khz = (uint64_t)khz;
     *out = c.release();
     return MFG_OK;
 }
@@ -1491,6 +1503,13 @@ int mfg_run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seed
     return guarded(c, [&] {
         if (!n_des) throw Err(MFG_ERR_INVALID, "null desorption counts");
         run_variants(c, n_des, seeds, max_events, r);
+    });
+}
+
+int mfg_set_time_limit(mfg_context* c, double seconds) {
+    return guarded(c, [&] {
+        if (!(seconds > 0.0 && seconds <= 86400.0)) throw Err(MFG_ERR_INVALID, "time limit must be in (0, 86400] s");
+        c->time_limit_s = seconds;
     });
 }
 

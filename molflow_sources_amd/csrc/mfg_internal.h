@@ -7,7 +7,7 @@
 //   double2 poly[]        normalised (u,v) polygon vertices
 //   BvhNode[]             up to 8-wide nodes, 128 B each, 8-bit quantised boxes
 //   LeafFacet[]           the ray/facet test data in leaf order (128 B) and
-//   LeafUV[]              triangle (u,v) polygons (64 B): a leaf is a
+//   LeafUV[]              triangle/quad (u,v) polygons (64 B): a leaf is a
 //                         contiguous run, no index indirection
 //   DVariant[nV]          per-variant bases and settings
 //   tallies               [variant][slot][facet] 64 B counters, textures,
@@ -145,11 +145,11 @@ struct alignas(128) LeafFacet {
     double invU2, invV2;
 };
 static_assert(sizeof(LeafFacet) == 128, "LeafFacet must be one 128 B line");
-// (u,v) of a triangle's vertices, parallel to LeafFacet (read only when the
-// ray meets the facet's (u,v) rectangle)
+// (u,v) of the vertices of a triangle or quadrilateral, parallel to
+// LeafFacet (read only when the ray meets the facet's (u,v) rectangle);
+// larger polygons are read from poly
 struct alignas(64) LeafUV {
-    double uv[6];
-    double pad[2];
+    double uv[8];
 };
 static_assert(sizeof(LeafUV) == 64, "LeafUV layout");
 
@@ -276,6 +276,11 @@ struct TraceParams {
     uint64_t total;                    // desorptions in this run
     uint64_t max_events;
     volatile int32_t* stop_flag;       // host-mapped
+    // a wave that has run for time_limit ticks of the device wall clock
+    // starts nothing more, parks (carry) or ends its particles, sets
+    // *expired and leaves: every launch ends even if no particle can
+    uint64_t time_limit;
+    unsigned long long* expired;
     // particles kept between launches: two halves of carry_cap records;
     // carry_ctl[0..1] count the records of each half, carry_ctl[2] names the
     // half this launch resumes from (the other half receives its leftovers)

@@ -183,6 +183,17 @@ __device__ __forceinline__ bool crossing(double ui, double vi, double uj, double
     }
     return false;
 }
+// in_poly for a quadrilateral held in registers (n = 4, same order)
+__device__ __forceinline__ bool in_quad(const LeafUV* __restrict__ LU, double u, double v) {
+    const double2* q = reinterpret_cast<const double2*>(LU->uv);
+    const double2 p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
+    bool inside = false;
+    if (crossing(p0.x, p0.y, p3.x, p3.y, u, v)) inside = !inside;
+    if (crossing(p1.x, p1.y, p0.x, p0.y, u, v)) inside = !inside;
+    if (crossing(p2.x, p2.y, p1.x, p1.y, u, v)) inside = !inside;
+    if (crossing(p3.x, p3.y, p2.x, p2.y, u, v)) inside = !inside;
+    return inside;
+}
 __device__ __forceinline__ bool in_tri(const LeafUV* __restrict__ LU, double u, double v) {
     const double2* q = reinterpret_cast<const double2*>(LU->uv);
     const double2 p0 = q[0], p1 = q[1], p2 = q[2];
@@ -216,8 +227,8 @@ __device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const d
 
 // facet_test on a leaf record: the same operations in the same order. The
 // record's 128 B are loaded together before the first decision (one round
-// trip); a triangle's (u,v) polygon is fetched only when the ray meets the
-// facet's rectangle.
+// trip); the (u,v) polygon of a triangle or quadrilateral is fetched only
+// when the ray meets the facet's rectangle.
 __device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const LeafUV* __restrict__ LU,
                                           const double* __restrict__ poly, d3 P, d3 D, int32_t skip, double& t,
                                           double& u, double& v, int32_t& id, uint32_t& flags) {
@@ -242,6 +253,7 @@ __device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const
               !((u < 0.0) | (u > 1.0) | (v < 0.0) | (v > 1.0)) & (id != skip) & (n >= 3u) & (hd.z != ~0u);
     if (!ok) return false;
     if (n == 3) return in_tri(LU, u, v);
+    if (n == 4) return in_quad(LU, u, v);
     return in_poly(poly + 2 * (size_t)hd.z, n, u, v);
 }
 
@@ -1479,9 +1491,16 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         n_out = P.carry_ctl + (half ^ 1u);
     }
     const uint64_t items = n_in + P.total;
+    const uint64_t t_start = wall_clock64();
+    uint32_t iter = 0;   // the loop is wave-uniform, and so is this count
     for (;;) {
+        // the wall clock (read every 64 iterations) is one value for the
+        // wave: expired is uniform
+        const bool expired = (++iter & 63u) == 0u &&
+            __builtin_amdgcn_readfirstlane((int)((uint64_t)(wall_clock64() - t_start) > P.time_limit)) != 0;
+        if (expired && lane == 0) atomicMax(P.expired, 1ull);
         bool need = !alive && !done;
-        uint64_t mask = __ballot(need);
+        uint64_t mask = expired ? 0ull : __ballot(need);
         bool dry = false;   // some lane of this wave found the queue exhausted
         if (mask) {
             uint32_t cnt = (uint32_t)__popcll(mask);
@@ -1566,10 +1585,11 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
                 if (started) T.desorption_records(src, m, ov);
             }
         }
-        if (P.carry_keep && dry) {
-            // the queue is empty: park this wave's particles for the next
-            // launch instead of running their tails here
-            uint64_t am = __ballot(alive);
+        if ((P.carry_keep && dry) || expired) {
+            // the queue is empty (or the time limit is reached): park this
+            // wave's particles for the next launch instead of running their
+            // tails here; without carry, expiry ends them where they are
+            uint64_t am = P.carry_keep ? __ballot(alive) : 0ull;
             if (am) {
                 uint32_t al = (uint32_t)__ffsll((unsigned long long)am) - 1;
                 unsigned long long ob = 0;

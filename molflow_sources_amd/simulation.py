@@ -112,6 +112,10 @@ class GpuSimulation:
     def request_stop(self) -> None:
         self._check(self._lib.mfg_request_stop(self.h), "mfg_request_stop")
 
+    def set_time_limit(self, seconds: float) -> None:
+        """Wall-clock limit of one trace launch; a run it ends reports stopped = 2."""
+        self._check(self._lib.mfg_set_time_limit(self.h, float(seconds)), "mfg_set_time_limit")
+
     def set_launch(self, block: int = 256, blocks_per_cu: int = 0) -> None:
         self._check(self._lib.mfg_set_launch(self.h, block, blocks_per_cu), "mfg_set_launch")
 

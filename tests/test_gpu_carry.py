@@ -150,6 +150,43 @@ def test_carry_stop_parks_particles(sim):
     assert g["nb_abs_equiv"] == r.desorbed
 
 
+def test_time_limit_ends_runs_whose_particles_never_end(sim):
+    # nothing absorbs and no event cap: only the launch time limit, forced
+    # short from the host, ends these runs
+    m = geometry.tube_model()
+    m.facets["sticking"][:2] = 0.0
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=4)
+    sim.set_time_limit(0.25)
+    try:
+        sim.set_carry(False)
+        r1 = sim.run(50_000_000, max_events=0)
+        assert r1.stopped == 2 and 0 < r1.desorbed < 50_000_000 and r1.hits > 0
+        assert sim.in_flight() == 0
+        sim.set_carry(True)
+        r2 = sim.run(50_000_000, max_events=0)
+        # every particle started is still alive, and parked
+        assert r2.stopped == 2 and 0 < r2.desorbed < 50_000_000
+        assert sim.in_flight() == r2.desorbed
+        r3 = sim.run(0, max_events=0)
+        assert r3.stopped == 2 and r3.desorbed == 0 and r3.hits > 0
+        assert sim.in_flight() == r2.desorbed
+        g = sim.read().glob[0]
+        assert g["nb_desorbed"] == r1.desorbed + r2.desorbed
+        assert g["nb_mc_hit"] == r1.hits + r2.hits + r3.hits
+        assert g["nb_abs_equiv"] == 0.0
+    finally:
+        sim.set_time_limit(60.0)
+        sim.set_carry(False)
+    # the default limit leaves an ordinary run alone
+    sim.load(geometry.tube_model())
+    sim.rebuild_accel()
+    sim.reset(seed=4)
+    r = sim.run(100_000, max_events=MAXEV)
+    assert r.stopped == 0 and r.desorbed == 100_000
+
+
 def test_carry_refused_with_adjoint(sim):
     from molflow_sources_amd.simulation import MolflowGpuError
     m = geometry.beam_pipe_model(n_az=24, n_ax=60, port_rings=3, n_textures=0, n_anglemaps=0)
