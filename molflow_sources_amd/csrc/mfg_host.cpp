@@ -1316,6 +1316,10 @@ void add_tallies(mfg_context* c, uint32_t vi, const mfg_tally_view* o) {
     uint64_t S = L.n_slots;
     if (o->facets) {
         auto h = fetch(c->fc.p, L.fc, S * L.n_facets);
+        // without low-flux weighting the kernel leaves the slot-0 hit
+        // equivalents to this read: every hit weighs 1
+        if (!c->dvars[vi].low_flux)
+            for (uint64_t i = 0; i < L.n_facets; ++i) h[i].nb_hit_equiv = (double)h[i].nb_mc_hit;
         for (size_t i = 0; i < h.size(); ++i) {
             auto& a = o->facets[i]; const auto& b = h[i];
             a.nb_mc_hit += b.nb_mc_hit; a.nb_hit_equiv += b.nb_hit_equiv; a.nb_desorbed += b.nb_desorbed;

@@ -721,13 +721,24 @@ struct Tracer {
         }
         (void)imp; (void)imp2; (void)impm;
     }
+    // Without low-flux weighting every particle weighs 1, so the slot-0
+    // nb_hit_equiv equals nb_mc_hit: it is not added here (one atomic less
+    // per hit) and the host fills it in when it reads the counters.
+    __device__ bool weighted() const {
+        if constexpr ((FEAT & FEAT_LOWFLUX) != 0) return var->low_flux != 0;
+        return false;
+    }
     // Atomics of this lane's pending slot-0 record, then clear it.
     __device__ void flush_pending_direct() {
         if (pend_f < 0) return;
         const DPhys& ph = P.phys[pend_f];
+        const bool w = weighted();
         if (ph.hot_idx >= 0) {
             double* h = hot + 8 * ph.hot_idx;   // same field order as mfg_facet_counters
-            if (pend_hits) { atomicAdd(reinterpret_cast<unsigned long long*>(h), (unsigned long long)pend_hits); atomic_addd(h + 1, pend_he); }
+            if (pend_hits) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(h), (unsigned long long)pend_hits);
+                if (w) atomic_addd(h + 1, pend_he);
+            }
             if (pend_abs != 0.0) atomic_addd(h + 3, pend_abs);
             if (pend_s1v != 0.0) atomic_addd(h + 4, pend_s1v);
             if (pend_svo != 0.0) atomic_addd(h + 5, pend_svo);
@@ -736,7 +747,10 @@ struct Tracer {
             return;
         }
         mfg_facet_counters* cc = P.fc + var->fc_base + (uint32_t)ph.local_id;
-        if (pend_hits) { atomic_addu(&cc->nb_mc_hit, pend_hits); atomic_addd(&cc->nb_hit_equiv, pend_he); }
+        if (pend_hits) {
+            atomic_addu(&cc->nb_mc_hit, pend_hits);
+            if (w) atomic_addd(&cc->nb_hit_equiv, pend_he);
+        }
         if (pend_abs != 0.0) atomic_addd(&cc->nb_abs_equiv, pend_abs);
         if (pend_s1v != 0.0) atomic_addd(&cc->sum_1_per_ort_velocity, pend_s1v);
         if (pend_svo != 0.0) atomic_addd(&cc->sum_v_ort, pend_svo);
@@ -1493,11 +1507,17 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     const uint64_t items = n_in + P.total;
     const uint64_t t_start = wall_clock64();
     uint32_t iter = 0;   // the loop is wave-uniform, and so is this count
+    // the stop flag lives in host memory: a read crosses the bus, so it is
+    // polled every 64 iterations, not at every refill
+    int stop = __builtin_amdgcn_readfirstlane(*P.stop_flag);
     for (;;) {
-        // the wall clock (read every 64 iterations) is one value for the
-        // wave: expired is uniform
-        const bool expired = (++iter & 63u) == 0u &&
-            __builtin_amdgcn_readfirstlane((int)((uint64_t)(wall_clock64() - t_start) > P.time_limit)) != 0;
+        // the wall clock and the stop flag (read every 64 iterations) are
+        // one value for the wave: expired and stop are uniform
+        bool expired = false;
+        if ((++iter & 63u) == 0u) {
+            expired = __builtin_amdgcn_readfirstlane((int)((uint64_t)(wall_clock64() - t_start) > P.time_limit)) != 0;
+            stop = __builtin_amdgcn_readfirstlane(*P.stop_flag);
+        }
         if (expired && lane == 0) atomicMax(P.expired, 1ull);
         bool need = !alive && !done;
         uint64_t mask = expired ? 0ull : __ballot(need);
@@ -1506,13 +1526,8 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
             uint32_t cnt = (uint32_t)__popcll(mask);
             uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
             unsigned long long base = 0;
-            int stop = 0;
-            if (lane == leader) {
-                stop = *P.stop_flag;
-                base = atomicAdd(P.next_particle, (unsigned long long)cnt);
-            }
+            if (lane == leader) base = atomicAdd(P.next_particle, (unsigned long long)cnt);
             base = __shfl(base, leader, 64);
-            stop = __shfl(stop, leader, 64);
             // after a stop request no new particle starts; resumed ones still
             // run (to the end, or back into the carry buffer)
             const uint64_t limit = stop ? n_in : items;
@@ -1613,7 +1628,9 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
             bool grp = T.pend_f == key;
             uint64_t gm = __ballot(grp);
             if (__popcll(gm) > 1) {
-                double he = wave_sum(grp ? T.pend_he : 0.0), ab = wave_sum(grp ? T.pend_abs : 0.0);
+                double he = 0.0;
+                if constexpr ((FEAT & FEAT_LOWFLUX) != 0) he = wave_sum(grp ? T.pend_he : 0.0);
+                double ab = wave_sum(grp ? T.pend_abs : 0.0);
                 double s1 = wave_sum(grp ? T.pend_s1v : 0.0), sv = wave_sum(grp ? T.pend_svo : 0.0);
                 double sp = wave_sum(grp ? T.pend_s1pv : 0.0);
                 unsigned long long hc = wave_sum_u(grp ? (unsigned long long)T.pend_hits : 0ull);
