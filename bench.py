@@ -12,6 +12,11 @@ drain launch that traces every remaining particle to its end, so every hit
 of the K*des-per-step desorptions is inside the timed region; the default
 100 steps of 1e6 is the full 1e8-desorption job.
 
+--workload tube runs the plain tube instead (R 1 cm, L 10 cm, 64 x 50 wall
+quads, cosine inlet, outlet of sticking 1): 10 steps of 1e6, the whole
+1e7-desorption job, with its transmission probability and its own CPU
+baseline.
+
 Prints ONE JSON line: hits/s (reference definition: bounces + absorptions +
 volume scatters per wall second) over K timed steps, with the roofline
 figure of the trace kernel (algorithmic bytes per hit from the reference-
@@ -59,19 +64,20 @@ def wall_rays(n_rays, seed=5, radius=4.0, length=200.0):
     return o, d
 
 
-def algorithmic_bytes_per_hit(model, tex_hit_fraction, n_rays=20000):
+def algorithmic_bytes_per_hit(model, tex_hit_fraction, n_rays=20000, radius=4.0, length=200.0):
     """Mean BVH nodes and facet tests per ray on the oracle's BVH, over
-    diffuse-bounce rays of the workload."""
+    diffuse-bounce rays of the workload (main cylinder of the given radius
+    and length)."""
     from oracle.mfcpu import CpuModel
     cm = CpuModel(model)
-    o, d = wall_rays(n_rays)
+    o, d = wall_rays(n_rays, radius=radius, length=length)
     nodes, prims = cm.bvh_stats(o, d)
     fields = FIELDS_PER_HIT + TEX_FIELDS * tex_hit_fraction
     b = NODE_BYTES * nodes + PRIM_BYTES * prims + FACET_BYTES + FIELD_BYTES * fields
     return b, nodes, prims
 
 
-def cpu_baseline(model, run_s=5.0, repeats=3):
+def cpu_baseline(model, run_s=5.0, repeats=3, name="beam-pipe"):
     """The CPU oracle (one tracer per OpenMP thread, thread-local tallies,
     the reference's CPU parallelism) on a bounded sample of the workload:
     a calibration run of at least a second sizes `repeats` runs of about
@@ -103,7 +109,7 @@ def cpu_baseline(model, run_s=5.0, repeats=3):
         total_s += dt
     rates.sort()
     return {"value": rates[len(rates) // 2], "unit": "hits/s", "cores": threads, "kind": "port",
-            "sample": f"median of {repeats} runs of {n_run} desorptions of the same beam-pipe model "
+            "sample": f"median of {repeats} runs of {n_run} desorptions of the same {name} model "
                       f"({total_hits} hits in {total_s:.1f} s in all), oracle/mfcpu.cpp",
             "runs": rates}
 
@@ -123,7 +129,8 @@ def traffic_from_profile():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed launches (default: the whole job, 100 for plain, 10 for tube)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--des-per-step", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -133,9 +140,10 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
-    ap.add_argument("--workload", choices=("plain", "batched", "adjoint"), default="plain",
-                    help="plain: the beam-pipe job (default); batched: many uneven variants in one launch "
-                         "vs one launch per variant; adjoint: sticking sensitivities vs a forward run")
+    ap.add_argument("--workload", choices=("plain", "tube", "batched", "adjoint"), default="plain",
+                    help="plain: the beam-pipe job (default); tube: the 1e7-desorption tube job; batched: "
+                         "many uneven variants in one launch vs one launch per variant; adjoint: sticking "
+                         "sensitivities vs a forward run")
     ap.add_argument("--variants", type=int, default=256)
     ap.add_argument("--variant-des", type=str, default="2e4,2e5",
                     help="desorptions per variant, log-uniform in [lo,hi]")
@@ -165,7 +173,17 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    model = geometry.beam_pipe_model()
+    tube = args.workload == "tube"
+    if args.steps is None:
+        args.steps = 10 if tube else 100
+    if tube:
+        # R 1 cm, L 10 cm, 64 x 50 wall quads, inlet disk desorbing (cosine),
+        # outlet disk of sticking 1, walls of sticking 0: 1e7 desorptions
+        model = geometry.tube_model()
+        ray_geom = {"radius": 0.998, "length": 10.0}   # just inside the 64-gon's facet planes
+    else:
+        model = geometry.beam_pipe_model()
+        ray_geom = {"radius": 4.0, "length": 200.0}
     sim = GpuSimulation(0)
     sim.load(model)
     sim.rebuild_accel()
@@ -205,13 +223,14 @@ def main():
     fh = t.facets["nb_mc_hit"][0].astype(float)
     tex_ids = list(getattr(model, "tex_ids", []))
     tex_frac = float(fh[tex_ids].sum() / max(fh.sum(), 1.0)) if tex_ids else 0.0
-    b_hit, nodes, prims = algorithmic_bytes_per_hit(model, tex_frac)
+    b_hit, nodes, prims = algorithmic_bytes_per_hit(model, tex_frac, **ray_geom)
     launches = args.steps + 1
     launch_s = kernel_ms / 1e3 / launches
     hits_per_launch = hits / launches
     achieved = b_hit * hits_per_launch / launch_s / 1e9
     out = {
-        "metric": "MC hits/s on 1 MI355X (plain tracer, config-2 beam pipe)",
+        "metric": "MC hits/s on 1 MI355X (plain tracer, config-1 tube)" if tube else
+                  "MC hits/s on 1 MI355X (plain tracer, config-2 beam pipe)",
         "value": value,
         "unit": "hits/s",
         "n_gpus": 1,
@@ -223,17 +242,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "beam_pipe_100k_triangles_8_ports_16_textures_4_anglemaps",
+        "config": {"workload": "tube_R1cm_L10cm_64x50_wall_quads" if tube else
+                               "beam_pipe_100k_triangles_8_ports_16_textures_4_anglemaps",
                    "facets": model.n_facets, "desorptions_per_step": args.des_per_step,
                    "hits_per_desorption": hits / (args.steps * args.des_per_step),
                    "seed": 42, "parallelism": "single GPU", "carry": not args.no_carry,
                    "drain_ms": drain_s * 1e3, "launch": tune, "job": "steps x desorptions_per_step, every particle traced to its end"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None if tube else traffic_from_profile(),
                      "bytes_per_hit": b_hit, "bvh_nodes_per_ray": nodes, "facet_tests_per_ray": prims,
                      "kernel_ms_per_launch": launch_s * 1e3},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(model),
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(model, name="tube" if tube else "beam-pipe"),
     }
+    if tube:
+        # facet 1 is the outlet disk: transmission = absorbed there / desorbed
+        out["config"]["transmission"] = float(t.facets["nb_abs_equiv"][0][1]) / float(t.glob[0]["nb_desorbed"])
     if out["cpu_baseline"]:
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)

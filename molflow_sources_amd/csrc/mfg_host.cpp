@@ -13,6 +13,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -458,6 +459,50 @@ void angle_source_block(std::vector<double>& out, const mfg_facet_desc& d, const
     }
 }
 
+// Slab index of a polygon of more than 4 vertices, stored right after its
+// normalised (u,v) list in the polygon array. The crossing-number test
+// toggles on every edge with exactly one end above v; sorted by v, the
+// distinct vertex heights w_0 < ... < w_{m-1} cut [w_0, w_{m-1}) into slabs,
+// and all v of one slab [w_k, w_{k+1}) see the same such edges: those with
+// min(v_i, v_j) <= w_k and max(v_i, v_j) >= w_{k+1}. A lookup finds v's slab
+// and evaluates the crossing predicate of the polygon loop, operand for
+// operand, on those edges only (two for a convex polygon); the parity of the
+// toggles does not depend on their order. Layout, in doubles: m (bits), the
+// m heights, m edge offsets (bits; slab k is [off_k, off_{k+1})), one pad,
+// then four doubles (u_i, v_i, u_j, v_j) per listed edge, j being i's
+// predecessor as in the loop; padded to an even count (the array holds
+// (u,v) pairs).
+void append_slabs(std::vector<double>& out, const std::vector<double>& pl) {
+    const size_t n = pl.size() / 2;
+    std::vector<double> w(n);
+    for (size_t i = 0; i < n; ++i) w[i] = pl[2 * i + 1];
+    std::sort(w.begin(), w.end());
+    w.erase(std::unique(w.begin(), w.end()), w.end());
+    const size_t m = w.size();
+    auto bits = [](uint64_t x) { double d; std::memcpy(&d, &x, sizeof d); return d; };
+    std::vector<double> edges;
+    std::vector<uint64_t> off(m, 0);
+    for (size_t k = 0; k + 1 < m; ++k) {
+        off[k] = edges.size() / 4;
+        for (size_t i = 0; i < n; ++i) {
+            const size_t j = i == 0 ? n - 1 : i - 1;
+            const double vi = pl[2 * i + 1], vj = pl[2 * j + 1];
+            if (std::min(vi, vj) <= w[k] && std::max(vi, vj) >= w[k + 1]) {
+                edges.push_back(pl[2 * i]); edges.push_back(vi);
+                edges.push_back(pl[2 * j]); edges.push_back(vj);
+            }
+        }
+    }
+    off[m - 1] = edges.size() / 4;
+    const size_t start = out.size();
+    out.push_back(bits(m));
+    out.insert(out.end(), w.begin(), w.end());
+    for (size_t k = 0; k < m; ++k) out.push_back(bits(off[k]));
+    out.push_back(0.0);   // the edges start on a 16 B boundary
+    out.insert(out.end(), edges.begin(), edges.end());
+    if ((out.size() - start) % 2) out.push_back(0.0);
+}
+
 void prepare(HostModel& M, const mfg_model_view* v) {
     if (!v) throw Err(MFG_ERR_INVALID, "null model view");
     if (v->struct_size != sizeof(mfg_model_view)) throw Err(MFG_ERR_INVALID, "mfg_model_view size mismatch");
@@ -547,6 +592,7 @@ void prepare(HostModel& M, const mfg_model_view* v) {
             pl[2 * i + 1] = (pv[i] - vmin) / (vmax - vmin);
         }
         M.poly.insert(M.poly.end(), pl.begin(), pl.end());
+        if (n > 4) append_slabs(M.poly, pl);
         double lu = norm(U), lv = norm(V);
         Box bx = empty_box();
         V3 c = {0, 0, 0};
@@ -768,6 +814,7 @@ struct mfg_context {
     uint32_t stack_size = 0;             // traversal stack entries a walk may need
     DevBuf<int32_t> stack_spill;         // per-lane entries beyond the LDS part
     int bvh_width = MFG_BVH_WIDTH, leaf_size = 2;   // children per BVH node, facets per leaf
+    uint32_t node_cache = 0;             // BVH nodes each block keeps in LDS (top of the breadth-first array)
     // device data
     DevBuf<DFacet> facets;
     DevBuf<DPhys> phys;
@@ -1039,6 +1086,35 @@ void zero_tallies(mfg_context* c) {
     c->in_flight = 0;   // the counters above include the carry control words
 }
 
+// Renumber a variant's nodes breadth-first from its roots, so the top of
+// every tree is one prefix of the array (the part a block copies to LDS).
+void breadth_first(std::vector<BvhNode>& nodes, std::vector<int32_t>& roots) {
+    std::vector<int32_t> order, idx(nodes.size(), -1);
+    order.reserve(nodes.size());
+    for (int32_t r : roots)
+        if (r >= 0) { idx[(size_t)r] = (int32_t)order.size(); order.push_back(r); }
+    for (size_t h = 0; h < order.size(); ++h) {
+        const BvhNode& nd = nodes[(size_t)order[h]];
+        for (int k = 0; k < 8; ++k) {
+            if (!(nd.valid & (1u << k)) || nd.count[k] > 0) continue;
+            int32_t ch = nd.child[k];
+            idx[(size_t)ch] = (int32_t)order.size();
+            order.push_back(ch);
+        }
+    }
+    if (order.size() != nodes.size()) throw Err(MFG_ERR_INVALID, "BVH nodes unreachable from the roots");
+    std::vector<BvhNode> out(nodes.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        BvhNode nd = nodes[(size_t)order[i]];
+        for (int k = 0; k < 8; ++k)
+            if ((nd.valid & (1u << k)) && nd.count[k] == 0) nd.child[k] = idx[(size_t)nd.child[k]];
+        out[i] = nd;
+    }
+    nodes.swap(out);
+    for (int32_t& r : roots)
+        if (r >= 0) r = idx[(size_t)r];
+}
+
 void build_accel(mfg_context* c, int width, int leaf) {
     if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
     if (width == 0) width = MFG_BVH_WIDTH;
@@ -1072,6 +1148,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
                     vroots[vi].push_back(r);
                     vdepth[vi] = std::max(vdepth[vi], col.max_depth);
                 }
+                breadth_first(vnodes[vi], vroots[vi]);
             } catch (const std::exception& e) {
                 errors[vi] = e.what();
             }
@@ -1131,6 +1208,12 @@ void build_accel(mfg_context* c, int width, int leaf) {
     reset_graph(c);
 }
 
+// Nodes read from LDS: the top of the tree of a single-variant model.
+uint32_t lds_node_count(const mfg_context* c) {
+    if (c->dvars.size() != 1) return 0;
+    return (uint32_t)std::min<size_t>(c->node_cache, c->nodes.n);
+}
+
 TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     TraceParams p{};
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
@@ -1155,6 +1238,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.expired = c->counters.p + 3;
     p.carry = c->carry.p; p.carry_ctl = c->counters.p + 4; p.carry_cap = c->carry_cap;
     p.carry_keep = c->carry_on ? 1u : 0u;
+    p.lds_nodes = lds_node_count(c);
     p.trace_facet = c->trace_f.p; p.trace_kind = c->trace_kind.p; p.trace_n = c->trace_n; p.trace_k = c->trace_k;
     p.adj_j0 = c->adj_j0.p; p.adj_jac = c->adj_jac.p; p.adj_sticking = c->adj_s.p; p.adj_counts = c->adj_counts.p;
     p.adj_groups = c->adj_groups; p.adj_targets = c->adj_targets;
@@ -1174,7 +1258,7 @@ void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& wa
     if (per_cu == 0) {
         // fill every CU to its occupancy limit (persistent launch)
         size_t lds = (size_t)block * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
-                     c->hot.size() * 8 * sizeof(double);
+                     c->hot.size() * 8 * sizeof(double) + (size_t)lds_node_count(c) * sizeof(BvhNode);
         int occ = mfg_trace_occupancy(feats, waves, block, lds);
         per_cu = (uint32_t)std::max(1, occ);
     }
@@ -1240,6 +1324,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     key.push_back((uint64_t)waves);
     key.push_back((uint64_t)(uintptr_t)p.carry); key.push_back(p.carry_keep);
     key.push_back((uint64_t)(uintptr_t)p.hot_fc); key.push_back(p.n_hot); key.push_back(p.time_limit);
+    key.push_back(p.lds_nodes);
     // the events bracket the trace kernel alone (eager path) or the whole
     // graph (replay path)
     auto enqueue = [&](bool events) {
@@ -1452,6 +1537,7 @@ int mfg_create(mfg_context** out, int device) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
         return MFG_ERR_DEVICE;
     c->clock_khz = (uint64_t)khz;
+    if (const char* e = std::getenv("MFG_NODE_CACHE")) c->node_cache = (uint32_t)std::atoi(e);
     *out = c.release();
     return MFG_OK;
 }

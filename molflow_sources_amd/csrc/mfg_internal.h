@@ -288,6 +288,9 @@ struct TraceParams {
     unsigned long long* carry_ctl;
     uint64_t carry_cap;
     uint32_t carry_keep;               // 1: leave particles in flight when the queue runs dry
+    // nodes [0, lds_nodes) (the top of the breadth-first node array) are
+    // read from a copy each block keeps in LDS
+    uint32_t lds_nodes;
     // trace
     int32_t* trace_facet;
     int32_t* trace_kind;

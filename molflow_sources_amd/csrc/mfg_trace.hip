@@ -174,6 +174,35 @@ __device__ __forceinline__ bool in_poly(const double* __restrict__ p, uint32_t n
     return inside;
 }
 
+// in_poly for more than 4 vertices through the slab index stored after the
+// (u,v) list: the same crossing predicate on the edges that straddle v
+// (exactly those the loop above toggles on), found by a binary search over
+// the sorted vertex heights.
+__device__ __forceinline__ uint32_t low_bits(double d) { return (uint32_t)__double_as_longlong(d); }
+__device__ bool in_poly_slabs(const double* __restrict__ p, uint32_t n, double u, double v) {
+    const double* S = p + 2 * (size_t)n;
+    const uint32_t m = low_bits(S[0]);
+    const double* w = S + 1;
+    if (!(v >= w[0] && v < w[m - 1])) return false;   // no straddling edge (NaN included)
+    uint32_t lo = 0, hi = m - 1;                       // w[lo] <= v < w[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (w[mid] <= v) lo = mid; else hi = mid;
+    }
+    const uint32_t e0 = low_bits(S[1 + m + lo]), e1 = low_bits(S[2 + m + lo]);
+    const double2* E = reinterpret_cast<const double2*>(S + 2 + 2 * (size_t)m);
+    bool inside = false;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const double2 a = E[2 * e], b = E[2 * e + 1];   // (u_i, v_i), (u_j, v_j)
+        const double x = (b.x - a.x) * (v - a.y) / (b.y - a.y) + a.x;
+        if (u < x) inside = !inside;
+    }
+    return inside;
+}
+__device__ __forceinline__ bool in_polygon(const double* __restrict__ p, uint32_t n, double u, double v) {
+    return n > 4 ? in_poly_slabs(p, n, u, v) : in_poly(p, n, u, v);
+}
+
 // in_poly for a triangle held in registers: the same crossing tests in the
 // same order as the loop above with n = 3.
 __device__ __forceinline__ bool crossing(double ui, double vi, double uj, double vj, double u, double v) {
@@ -222,7 +251,7 @@ __device__ __forceinline__ bool facet_test(const DFacet* __restrict__ F, const d
     u = fdot(r, ld3(F->U)) * F->invU2;
     v = fdot(r, ld3(F->V)) * F->invV2;
     if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return false;
-    return in_poly(poly + 2 * (size_t)F->poly_off, F->poly_n, u, v);
+    return in_polygon(poly + 2 * (size_t)F->poly_off, F->poly_n, u, v);
 }
 
 // facet_test on a leaf record: the same operations in the same order. The
@@ -254,7 +283,7 @@ __device__ __forceinline__ bool leaf_test(const LeafFacet* __restrict__ L, const
     if (!ok) return false;
     if (n == 3) return in_tri(LU, u, v);
     if (n == 4) return in_quad(LU, u, v);
-    return in_poly(poly + 2 * (size_t)hd.z, n, u, v);
+    return in_poly_slabs(poly + 2 * (size_t)hd.z, n, u, v);
 }
 
 struct Ray32 { float ox, oy, oz, ix, iy, iz; };
@@ -282,6 +311,18 @@ __device__ __forceinline__ NodeRegs load_node(const BvhNode* __restrict__ nd) {
     NodeRegs n;
     n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6];
     return n;
+}
+
+// A node from the block's LDS copy of the top of the tree, else from HBM.
+__device__ __forceinline__ NodeRegs load_node_at(const BvhNode* __restrict__ nodes, const uint4* lnodes,
+                                                 uint32_t n_lds, int32_t node) {
+    if ((uint32_t)node < n_lds) {
+        const uint4* q = lnodes + (size_t)node * 8;
+        NodeRegs n;
+        n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6];
+        return n;
+    }
+    return load_node(nodes + node);
 }
 
 __device__ __forceinline__ uint32_t node_valid(const NodeRegs& n) { return n.c0.w >> 24; }
@@ -513,7 +554,8 @@ struct LaneStack {
 template <bool TRANSP>
 __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int32_t skip, double tmax,
                            const LaneStack& S, const Rng* rng,
-                           uint32_t event, const DVariant* var, double time, bool opaque_only) {
+                           uint32_t event, const DVariant* var, double time, bool opaque_only,
+                           const uint4* lnodes) {
     HitRec h{-1, tmax, 0.0, 0.0};
     if (root < 0) return h;
     Ray32 r = make_ray32(Pp, D);
@@ -521,7 +563,7 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     uint32_t sp = 0;
     int32_t node = root;
     for (;;) {
-        const NodeRegs nd = load_node(P.nodes + node);
+        const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
         float tn[8];
         uint32_t m = node_hits(nd, r, tbf, tn);
         // leaf children first (one loop for all slots, so lanes whose leaves
@@ -610,6 +652,7 @@ struct Tracer {
     uint32_t tr_n;
     uint32_t* adj_cnt;
     double* hot;                      // LDS table of the hot facets' slot-0 counters (8 words each)
+    const uint4* lnodes;              // LDS copy of the first P.lds_nodes BVH nodes
     // pending slot-0 facet counter update of this lane
     int32_t pend_f;
     uint32_t pend_hits;
@@ -623,6 +666,7 @@ struct Tracer {
         adj_cnt = nullptr;
         var = nullptr;
         hot = nullptr;
+        lnodes = nullptr;
         clear_pending();
     }
 
@@ -1061,7 +1105,7 @@ struct Tracer {
                 u = rng.rnd();
                 v = rng.rnd();
             }
-            if (in_poly(poly, pn, u, v)) {
+            if (in_polygon(poly, pn, u, v)) {
                 pos = add(add(O, scl(u, U)), scl(v, V));
                 hu = u; hv = v;
                 found = true;
@@ -1279,10 +1323,10 @@ struct Tracer {
         d3 O = ld3(D->O), U = ld3(D->U), V = ld3(D->V);
         pos = add(add(O, scl(u, U)), scl(v, V));
         const double* poly = P.poly + 2 * (size_t)D->poly_off;
-        if (!in_poly(poly, D->poly_n, u, v)) {
+        if (!in_polygon(poly, D->poly_n, u, v)) {
             for (int tr = 0; tr < 1000; ++tr) {
                 u = rng.rnd(); v = rng.rnd();
-                if (in_poly(poly, D->poly_n, u, v)) { pos = add(add(O, scl(u, U)), scl(v, V)); break; }
+                if (in_polygon(poly, D->poly_n, u, v)) { pos = add(add(O, scl(u, U)), scl(v, V)); break; }
             }
         }
         last_hit = dg;
@@ -1360,7 +1404,7 @@ struct Tracer {
         ++nb_events;
         int32_t root = P.roots[var->root_base + (uint32_t)structure];
         HitRec h = traverse<TR>(P, root, pos, dir, last_hit, 1e300, S, &rng, nb_events, var,
-                                time, false);
+                                time, false, lnodes);
         if (h.facet < 0) {  // leak
             ++acc_leak;
             unsigned long long at = atomicAdd(P.leak_count, 1ull);
@@ -1483,8 +1527,16 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     // hot-facet counters after the stacks (stack bytes are a multiple of 8)
     double* hot = reinterpret_cast<double*>(lds_stack + (size_t)blockDim.x * P.stack_size);
     for (uint32_t i = threadIdx.x; i < 8u * P.n_hot; i += blockDim.x) hot[i] = 0.0;
+    // then the block's copy of the top of the tree (128 B per node; the
+    // offsets above are multiples of 64 B)
+    uint4* lnodes = reinterpret_cast<uint4*>(hot + 8u * P.n_hot);
+    {
+        const uint4* g = reinterpret_cast<const uint4*>(P.nodes);
+        for (uint32_t i = threadIdx.x; i < 8u * P.lds_nodes; i += blockDim.x) lnodes[i] = g[i];
+    }
     __syncthreads();
     T.hot = hot;
+    T.lnodes = lnodes;
     if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
         T.adj_cnt = P.adj_counts + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * P.adj_groups;
     }
@@ -1711,7 +1763,7 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     // the whole stack is in LDS here (cap == lds_cap): the spill pointer is never used
     const LaneStack S{lds_stack + threadIdx.x, blockDim.x, p.stack_size, p.stack_size, lds_stack + threadIdx.x};
     HitRec h = traverse<false>(tp, p.root, P, D, skip, 1e300, S,
-                               nullptr, 0, nullptr, 0.0, true);
+                               nullptr, 0, nullptr, 0.0, true, nullptr);
     p.out_facet[i] = h.facet >= 0 ? h.facet - (int32_t)p.facet_base : -1;
     p.out_t[i] = h.facet >= 0 ? h.t : 0.0;
     p.out_u[i] = h.u;
@@ -1791,7 +1843,8 @@ int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds) {
 }
 
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream) {
-    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t) + (size_t)p.n_hot * 8 * sizeof(double);
+    size_t lds = (size_t)block * p.stack_size * sizeof(int32_t) + (size_t)p.n_hot * 8 * sizeof(double) +
+                 (size_t)p.lds_nodes * sizeof(BvhNode);
     TraceKernel k = kernel_for(mfg_trace_kernel_features(features), waves);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;

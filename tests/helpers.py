@@ -85,3 +85,34 @@ def read_driver_output(path):
     hits = np.array([r[0] for r in rows], dtype=np.int64)
     absq = np.array([r[1] for r in rows])
     return head, hits, absq
+
+
+def polygon_zoo_model():
+    """Four large planar polygons stacked along z (2-sided, sticking 0.3):
+    an 11-pointed star (non-convex, 22 vertices, the desorbing source), an
+    axis-aligned comb (equal vertex heights and horizontal edges), a 64-gon
+    and a C shape. They exercise the point-in-polygon test of polygons with
+    more than 4 vertices, in intersections and in desorption sampling."""
+    from molflow_sources_amd._abi import DES_COSINE
+    from molflow_sources_amd.model import Model, default_facets
+
+    k = 11
+    star = [((1.0 if i % 2 == 0 else 0.45) * math.cos(math.pi * i / k),
+             (1.0 if i % 2 == 0 else 0.45) * math.sin(math.pi * i / k)) for i in range(2 * k)]
+    comb = [(0, 0), (4, 0), (4, 1), (3.5, 1), (3.5, 0.4), (2.5, 0.4), (2.5, 1), (1.5, 1), (1.5, 0.4),
+            (0.5, 0.4), (0.5, 1), (0, 1)]
+    comb = [(0.5 * x - 1.0, 1.5 * y - 0.75) for x, y in comb]
+    disk = [(math.cos(2 * math.pi * i / 64), math.sin(2 * math.pi * i / 64)) for i in range(64)]
+    c_shape = ([(math.cos(a), math.sin(a)) for a in np.linspace(0.4, 2 * math.pi - 0.4, 10)] +
+               [(0.5 * math.cos(a), 0.5 * math.sin(a)) for a in np.linspace(2 * math.pi - 0.4, 0.4, 10)])
+    verts, polys = [], []
+    for z, shape in enumerate((star, comb, disk, c_shape)):
+        base = len(verts)
+        verts += [(x, y, float(z)) for x, y in shape]
+        polys.append(list(range(base, base + len(shape))))
+    F = default_facets(len(polys))
+    F["is_2sided"] = 1
+    F["sticking"] = 0.3
+    F["outgassing"][0] = 1.0
+    F["desorb_type"][0] = DES_COSINE
+    return Model(np.array(verts, dtype=float), polys, F, name="polygon_zoo")
