@@ -13,7 +13,6 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -814,7 +813,10 @@ struct mfg_context {
     uint32_t stack_size = 0;             // traversal stack entries a walk may need
     DevBuf<int32_t> stack_spill;         // per-lane entries beyond the LDS part
     int bvh_width = MFG_BVH_WIDTH, leaf_size = 2;   // children per BVH node, facets per leaf
-    uint32_t node_cache = 0;             // BVH nodes each block keeps in LDS (top of the breadth-first array)
+    // BVH nodes each block keeps in LDS (top of the breadth-first array):
+    // 16 KB, which leaves 256-thread blocks their occupancy up to 5 waves
+    // per SIMD
+    uint32_t node_cache = 128;
     // device data
     DevBuf<DFacet> facets;
     DevBuf<DPhys> phys;
@@ -1537,7 +1539,6 @@ int mfg_create(mfg_context** out, int device) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
         return MFG_ERR_DEVICE;
     c->clock_khz = (uint64_t)khz;
-    if (const char* e = std::getenv("MFG_NODE_CACHE")) c->node_cache = (uint32_t)std::atoi(e);
     *out = c.release();
     return MFG_OK;
 }

@@ -338,25 +338,27 @@ __device__ __forceinline__ uint32_t node_count(const NodeRegs& n, uint32_t c) {
 __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
 // Slab test of the node's children against [0, tmax]: hit mask + entry t.
-// The quantised boxes enclose the padded FP64 boxes, so the FP32 test never
-// drops a box that holds an FP64 hit.
+// A bound q (8 bits) on an axis sits at o + q 2^e, so its ray parameter is
+// (o + q 2^e - ray_o) / d = fma(q, 2^e/d, (o - ray_o)/d): two per-node
+// constants per axis, then one fma per bound. The quantised boxes enclose
+// the FP64 boxes padded by 1e-5 of the model extent, far above the FP32
+// rounding of these steps, so the test never drops a box that holds an
+// FP64 hit.
 __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[8]) {
     const float ox = __uint_as_float(n.c0.x), oy = __uint_as_float(n.c0.y), oz = __uint_as_float(n.c0.z);
     const uint32_t ew = n.c0.w;
-    const float sx = exp2i((int)(int8_t)(ew & 0xffu)), sy = exp2i((int)(int8_t)((ew >> 8) & 0xffu));
-    const float sz = exp2i((int)(int8_t)((ew >> 16) & 0xffu));
+    const float ax = exp2i((int)(int8_t)(ew & 0xffu)) * r.ix, ay = exp2i((int)(int8_t)((ew >> 8) & 0xffu)) * r.iy;
+    const float az = exp2i((int)(int8_t)((ew >> 16) & 0xffu)) * r.iz;
+    const float bx = (ox - r.ox) * r.ix, by = (oy - r.oy) * r.iy, bz = (oz - r.oz) * r.iz;
     uint32_t mask = 0;
 #pragma unroll
     for (uint32_t c = 0; c < 8; ++c) {
         const uint32_t sh = 8u * (c & 3u);
         const uint32_t lxw = c < 4 ? n.c3.z : n.c3.w, lyw = c < 4 ? n.c4.x : n.c4.y, lzw = c < 4 ? n.c4.z : n.c4.w;
         const uint32_t hxw = c < 4 ? n.c5.x : n.c5.y, hyw = c < 4 ? n.c5.z : n.c5.w, hzw = c < 4 ? n.c6.x : n.c6.y;
-        const float lx = fmaf((float)((lxw >> sh) & 0xffu), sx, ox), hx = fmaf((float)((hxw >> sh) & 0xffu), sx, ox);
-        const float ly = fmaf((float)((lyw >> sh) & 0xffu), sy, oy), hy = fmaf((float)((hyw >> sh) & 0xffu), sy, oy);
-        const float lz = fmaf((float)((lzw >> sh) & 0xffu), sz, oz), hz = fmaf((float)((hzw >> sh) & 0xffu), sz, oz);
-        float tx0 = (lx - r.ox) * r.ix, tx1 = (hx - r.ox) * r.ix;
-        float ty0 = (ly - r.oy) * r.iy, ty1 = (hy - r.oy) * r.iy;
-        float tz0 = (lz - r.oz) * r.iz, tz1 = (hz - r.oz) * r.iz;
+        float tx0 = fmaf((float)((lxw >> sh) & 0xffu), ax, bx), tx1 = fmaf((float)((hxw >> sh) & 0xffu), ax, bx);
+        float ty0 = fmaf((float)((lyw >> sh) & 0xffu), ay, by), ty1 = fmaf((float)((hyw >> sh) & 0xffu), ay, by);
+        float tz0 = fmaf((float)((lzw >> sh) & 0xffu), az, bz), tz1 = fmaf((float)((hzw >> sh) & 0xffu), az, bz);
         float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
         tn[c] = a;
@@ -533,9 +535,11 @@ struct LaneStack {
     int32_t* lds;
     uint32_t stride, lds_cap, cap;
     int32_t* spill;
+    // entry sp of a lane sits sp * stride words after its first one (both
+    // below 2^24: a full-rate 24-bit multiply)
     __device__ __forceinline__ void push(uint32_t& sp, int32_t v) const {
         if (sp < lds_cap) {
-            lds[(size_t)sp * stride] = v;
+            lds[__umul24(sp, stride)] = v;
             ++sp;
         } else if (sp < cap) {
             spill[sp - lds_cap] = v;
@@ -544,7 +548,7 @@ struct LaneStack {
     }
     __device__ __forceinline__ int32_t pop(uint32_t& sp) const {
         --sp;
-        return sp < lds_cap ? lds[(size_t)sp * stride] : spill[sp - lds_cap];
+        return sp < lds_cap ? lds[__umul24(sp, stride)] : spill[sp - lds_cap];
     }
 };
 
