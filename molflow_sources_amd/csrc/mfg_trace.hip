@@ -350,17 +350,25 @@ __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r,
     const float ax = exp2i((int)(int8_t)(ew & 0xffu)) * r.ix, ay = exp2i((int)(int8_t)((ew >> 8) & 0xffu)) * r.iy;
     const float az = exp2i((int)(int8_t)((ew >> 16) & 0xffu)) * r.iz;
     const float bx = (ox - r.ox) * r.ix, by = (oy - r.oy) * r.iy, bz = (oz - r.oz) * r.iz;
+    // fma(q, a, b) is monotonic in q, rising when the axis' 1/d is >= 0: the
+    // low bound is then the entry plane and the high one the exit plane, and
+    // the reverse otherwise (an empty slot, low 255 > high 0, still misses)
+    const bool px = r.ix >= 0.0f, py = r.iy >= 0.0f, pz = r.iz >= 0.0f;
+    const uint32_t nx[2] = {px ? n.c3.z : n.c5.x, px ? n.c3.w : n.c5.y};
+    const uint32_t fx[2] = {px ? n.c5.x : n.c3.z, px ? n.c5.y : n.c3.w};
+    const uint32_t ny[2] = {py ? n.c4.x : n.c5.z, py ? n.c4.y : n.c5.w};
+    const uint32_t fy[2] = {py ? n.c5.z : n.c4.x, py ? n.c5.w : n.c4.y};
+    const uint32_t nz[2] = {pz ? n.c4.z : n.c6.x, pz ? n.c4.w : n.c6.y};
+    const uint32_t fz[2] = {pz ? n.c6.x : n.c4.z, pz ? n.c6.y : n.c4.w};
     uint32_t mask = 0;
 #pragma unroll
     for (uint32_t c = 0; c < 8; ++c) {
-        const uint32_t sh = 8u * (c & 3u);
-        const uint32_t lxw = c < 4 ? n.c3.z : n.c3.w, lyw = c < 4 ? n.c4.x : n.c4.y, lzw = c < 4 ? n.c4.z : n.c4.w;
-        const uint32_t hxw = c < 4 ? n.c5.x : n.c5.y, hyw = c < 4 ? n.c5.z : n.c5.w, hzw = c < 4 ? n.c6.x : n.c6.y;
-        float tx0 = fmaf((float)((lxw >> sh) & 0xffu), ax, bx), tx1 = fmaf((float)((hxw >> sh) & 0xffu), ax, bx);
-        float ty0 = fmaf((float)((lyw >> sh) & 0xffu), ay, by), ty1 = fmaf((float)((hyw >> sh) & 0xffu), ay, by);
-        float tz0 = fmaf((float)((lzw >> sh) & 0xffu), az, bz), tz1 = fmaf((float)((hzw >> sh) & 0xffu), az, bz);
-        float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const uint32_t sh = 8u * (c & 3u), h = c >> 2;
+        const float tnx = fmaf((float)((nx[h] >> sh) & 0xffu), ax, bx), tfx = fmaf((float)((fx[h] >> sh) & 0xffu), ax, bx);
+        const float tny = fmaf((float)((ny[h] >> sh) & 0xffu), ay, by), tfy = fmaf((float)((fy[h] >> sh) & 0xffu), ay, by);
+        const float tnz = fmaf((float)((nz[h] >> sh) & 0xffu), az, bz), tfz = fmaf((float)((fz[h] >> sh) & 0xffu), az, bz);
+        float a = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+        float b = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         tn[c] = a;
         if (a <= b * 1.000001f) mask |= 1u << c;
     }
