@@ -610,34 +610,15 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
                 }
             }
         }
-        // inner children that survive the (possibly shrunk) tmax, sorted
-        // near-to-far with a fixed network (registers only)
-        constexpr float INF = 3.4e38f;
-        float k[8];
-        int32_t cid[8];
+        // inner children that survive the (possibly shrunk) tmax, pushed in
+        // slot order; the next node is the top of the stack. A near-to-far
+        // sort of the children leaves the node visits per ray within 2 % on
+        // these trees but costs about a hundred instructions per node, and
+        // the nearest hit does not depend on the visiting order (ties go to
+        // the lowest facet id)
 #pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            const bool in = (im & (1u << c)) && !(tn[c] > tbf);
-            k[c] = in ? tn[c] : INF;
-            cid[c] = in ? node_child(nd, c) : -1;
-        }
-#define MFG_CSWAP(a, b) \
-    if (k[b] < k[a]) { float tk = k[a]; k[a] = k[b]; k[b] = tk; int32_t tc = cid[a]; cid[a] = cid[b]; cid[b] = tc; }
-        // Batcher odd-even merge sort of 8 keys (19 compare-exchanges)
-        MFG_CSWAP(0, 1) MFG_CSWAP(2, 3) MFG_CSWAP(4, 5) MFG_CSWAP(6, 7)
-        MFG_CSWAP(0, 2) MFG_CSWAP(1, 3) MFG_CSWAP(4, 6) MFG_CSWAP(5, 7)
-        MFG_CSWAP(1, 2) MFG_CSWAP(5, 6)
-        MFG_CSWAP(0, 4) MFG_CSWAP(1, 5) MFG_CSWAP(2, 6) MFG_CSWAP(3, 7)
-        MFG_CSWAP(2, 4) MFG_CSWAP(3, 5)
-        MFG_CSWAP(1, 2) MFG_CSWAP(3, 4) MFG_CSWAP(5, 6)
-#undef MFG_CSWAP
-        if (cid[0] >= 0) {
-#pragma unroll
-            for (int c = 7; c >= 1; --c)
-                if (cid[c] >= 0) S.push(sp, cid[c]);
-            node = cid[0];
-            continue;
-        }
+        for (int c = 7; c >= 0; --c)
+            if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
         if (sp == 0) break;
         node = S.pop(sp);
     }
