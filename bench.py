@@ -353,17 +353,6 @@ def adjoint_bench(args):
     up = np.random.default_rng(42).normal(size=len(tg))
     max_events = 50_000_000
     n = args.adjoint_des
-    # forward cost: same model, same sticking, no Jacobian
-    f = GpuSimulation(0)
-    f.load(m)
-    f.rebuild_accel()
-    tf = f.autotune(max(n // 4, 10_000))
-    f.reset(seed=42)
-    f.run(min(n, 20_000), max_events=max_events)
-    f.reset(seed=42)
-    rf = f.run(n, max_events=max_events)
-    f.close()
-    progress(f"forward: {rf.hits} hits in {rf.kernel_ms:.1f} ms")
     a = GpuSimulation(0)
     a.load(m)
     a.rebuild_accel()
@@ -376,6 +365,28 @@ def adjoint_bench(args):
     ra = a.run(n, max_events=max_events)
     grad = a.adjoint_backward(up)
     progress(f"adjoint: {ra.hits} hits in {ra.kernel_ms:.1f} ms")
+    # forward cost: same model, same sticking, no Jacobian, timed with its
+    # own autotuned launch and with the adjoint's; the faster is the baseline
+    f = GpuSimulation(0)
+    f.load(m)
+    f.rebuild_accel()
+    tf = f.autotune(max(n // 4, 10_000))
+    shapes = [(tf.waves, tf.block, tf.blocks_per_cu, tf.bvh_width), (ta.waves, ta.block, ta.blocks_per_cu, ta.bvh_width)]
+    fwd = []
+    for w, blk, per_cu, width in shapes:
+        f.rebuild_accel(width)
+        f.set_kernel_variant(w)
+        f.set_launch(blk, per_cu)
+        f.reset(seed=42)
+        f.run(min(n, 20_000), max_events=max_events)
+        f.reset(seed=42)
+        fwd.append(f.run(n, max_events=max_events))
+    f.close()
+    k = min(range(len(fwd)), key=lambda i: fwd[i].kernel_ms)
+    rf = fwd[k]
+    tf_used = shapes[k]
+    progress(f"forward: {rf.hits} hits in {rf.kernel_ms:.1f} ms (launches tried: "
+             + ", ".join(f"{r.kernel_ms:.1f} ms" for r in fwd) + ")")
     # finite differences with common random numbers, in batches for sigma
     nb, per = 8, max(1, args.fd_des // 8)
 
@@ -418,7 +429,8 @@ def adjoint_bench(args):
                      "max_rel_diff": float(np.max(np.abs(fd_b.mean(0) - adj_b.mean(0)) / np.abs(adj_b.mean(0)))),
                      "crn_difference_z": [float(x) for x in z_crn],
                      "batches": nb, "desorptions_per_batch": per},
-        "launch": {"forward": [tf.waves, tf.block], "adjoint": [ta.waves, ta.block]},
+        "launch": {"forward": list(tf_used[:2]), "adjoint": [ta.waves, ta.block],
+                   "forward_kernel_ms_tried": [r.kernel_ms for r in fwd]},
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "beam_pipe_low_flux_adjoint", "facets": m.n_facets, "desorptions": n,
                    "groups": len(base), "targets": len(tg), "low_flux_cutoff": 1e-6},
