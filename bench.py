@@ -196,7 +196,7 @@ def main():
         # its probe runs: it zeroes the tallies afterwards)
         tr = sim.autotune(args.des_per_step)
         tune = {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu,
-                "bvh_width": tr.bvh_width, "hot_facets": tr.n_hot, "probe_hits_per_s": tr.hits_per_s}
+                "bvh_width": tr.bvh_width, "leaf_size": tr.leaf_size, "hot_facets": tr.n_hot, "probe_hits_per_s": tr.hits_per_s}
     max_events = 50_000_000
     for _ in range(args.warmup):
         sim.run(args.des_per_step, max_events=max_events)
@@ -371,10 +371,11 @@ def adjoint_bench(args):
     f.load(m)
     f.rebuild_accel()
     tf = f.autotune(max(n // 4, 10_000))
-    shapes = [(tf.waves, tf.block, tf.blocks_per_cu, tf.bvh_width), (ta.waves, ta.block, ta.blocks_per_cu, ta.bvh_width)]
+    shapes = [(tf.waves, tf.block, tf.blocks_per_cu, tf.bvh_width, tf.leaf_size),
+              (ta.waves, ta.block, ta.blocks_per_cu, ta.bvh_width, ta.leaf_size)]
     fwd = []
-    for w, blk, per_cu, width in shapes:
-        f.rebuild_accel(width)
+    for w, blk, per_cu, width, leaf in shapes:
+        f.rebuild_accel(width, leaf)
         f.set_kernel_variant(w)
         f.set_launch(blk, per_cu)
         f.reset(seed=42)

@@ -398,8 +398,9 @@ int mfg_in_flight(const mfg_context* ctx, uint64_t* n);
 int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);
 
 /* Launch autotuner: times short probe runs of n_probe desorptions for every
- * BVH width (8, 4), compiled occupancy bound and block size on the loaded
- * model(s) (particles stay in flight between probes), keeps the
+ * BVH shape (8-wide with 2 or 1 facets per leaf, 4-wide with 2), compiled
+ * occupancy bound and block size on the loaded model(s) (particles stay in
+ * flight between probes), keeps the
  * fastest, then zeroes the tallies and puts the particle streams back where
  * they were, so the probes leave no trace in later results. */
 typedef struct mfg_tune_result {
@@ -410,7 +411,7 @@ typedef struct mfg_tune_result {
                                 (those with >= 0.5 % of the probes' counter updates) */
     double hits_per_s;       /* probe throughput of the choice */
     uint32_t bvh_width;      /* children per BVH node of the choice (the BVH is rebuilt with it) */
-    uint32_t reserved;
+    uint32_t leaf_size;      /* facets per BVH leaf of the choice */
 } mfg_tune_result;
 int mfg_autotune(mfg_context* ctx, uint64_t n_probe, mfg_tune_result* out);
 

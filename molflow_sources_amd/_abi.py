@@ -99,7 +99,7 @@ class TallyView(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("waves", _u32), ("block", _u32), ("blocks_per_cu", _u32), ("n_hot", _u32), ("hits_per_s", _d),
-                ("bvh_width", _u32), ("reserved", _u32)]
+                ("bvh_width", _u32), ("leaf_size", _u32)]
 
 
 class StepResult(C.Structure):

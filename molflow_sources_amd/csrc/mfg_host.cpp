@@ -1789,10 +1789,13 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
         c->carry_on = c->adj_groups == 0;
         double best = -1;
         mfg_tune_result r{};
-        const int widths[] = {8, 4};
-        for (int width : widths) {
-            if (width != c->bvh_width) build_accel(c, width, c->leaf_size);
-            // one untimed probe per width loads the code objects and fills the lanes
+        // BVH shapes tried: 8-wide with 2 and 1 facets per leaf, 4-wide with 2
+        // (a leaf of one facet needs no per-leaf loop; more nodes in exchange)
+        const int shapes[][2] = {{8, 2}, {8, 1}, {4, 2}};
+        for (const auto& sh : shapes) {
+            const int width = sh[0], leaf = sh[1];
+            if (width != c->bvh_width || leaf != c->leaf_size) build_accel(c, width, leaf);
+            // one untimed probe per shape loads the code objects and fills the lanes
             c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
             run_variants(c, nd.data(), seeds.data(), 0, nullptr);
             for (int i = 0; i < nw; ++i)
@@ -1806,11 +1809,12 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
                         int grid, block, waves;
                         launch_shape(c, feats, grid, block, waves);
                         r.waves = (uint32_t)waves; r.block = b; r.blocks_per_cu = (uint32_t)(grid / c->n_cu);
-                        r.hits_per_s = hs; r.bvh_width = (uint32_t)width;
+                        r.hits_per_s = hs; r.bvh_width = (uint32_t)width; r.leaf_size = (uint32_t)leaf;
                     }
                 }
         }
-        if ((int)r.bvh_width != c->bvh_width) build_accel(c, (int)r.bvh_width, c->leaf_size);
+        if ((int)r.bvh_width != c->bvh_width || (int)r.leaf_size != c->leaf_size)
+            build_accel(c, (int)r.bvh_width, (int)r.leaf_size);
         c->carry_on = save_carry;
         c->waves = (int)r.waves; c->block = r.block; c->blocks_per_cu = 0;
         // hot facets: those that took at least 1/2 % of the probes' slot-0

@@ -222,20 +222,20 @@ def test_adjoint_matches_central_differences(sim):
 
 
 def test_bvh_width_and_autotune_do_not_change_results(sim):
-    """The traversal order (BVH width 4 or 8), the kernel variant, the launch
+    """The traversal order (BVH width 4 or 8, leaves of 1 or 2 facets), the kernel variant, the launch
     shape and the hot-facet privatisation chosen by the autotuner change no
     particle's events: integer tallies are identical for one seed."""
     m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=4, n_anglemaps=2)
     sim.load(m)
     res = []
-    for width in (8, 4):
-        sim.rebuild_accel(width, 2)
+    for width, leaf in ((8, 2), (4, 2), (8, 1)):
+        sim.rebuild_accel(width, leaf)
         sim.reset(seed=12)
         sim.run(20_000, max_events=MAXEV)
         t = sim.read()
         res.append(int_tallies(t) + (t.angle_maps.copy(),))
     t = sim.autotune(20_000)
-    assert t.bvh_width in (4, 8) and t.hits_per_s > 0
+    assert t.bvh_width in (4, 8) and t.leaf_size in (1, 2) and t.hits_per_s > 0
     sim.reset(seed=12)
     sim.run(20_000, max_events=MAXEV)
     tt = sim.read()
