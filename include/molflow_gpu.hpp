@@ -44,7 +44,7 @@ public:
     void LoadSimulation(const mfg_model_view& view) { check(mfg_load_model(ctx_, &view), "mfg_load_model"); }
 
     /* RebuildAccelStructure (MolflowSimulation.h:30) */
-    int RebuildAccelStructure(int width = 4, int leafSize = 2) {
+    int RebuildAccelStructure(int width = 8, int leafSize = 2) {
         check(mfg_build_accel(ctx_, width, leafSize), "mfg_build_accel");
         return 0;
     }

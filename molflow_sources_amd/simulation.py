@@ -86,7 +86,7 @@ class GpuSimulation:
         self._check(self._lib.mfg_load_variants(self.h, arr, len(views)), "mfg_load_variants")
         self.models, self.model = list(models), None
 
-    def rebuild_accel(self, width: int = 4, leaf_size: int = 2) -> None:
+    def rebuild_accel(self, width: int = 8, leaf_size: int = 2) -> None:
         self._check(self._lib.mfg_build_accel(self.h, width, leaf_size), "mfg_build_accel")
 
     def reset(self, seed: int = 42, particle_offset: int = 0) -> None:
