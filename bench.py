@@ -126,6 +126,39 @@ def traffic_from_profile():
     return None
 
 
+def atomic_peak(device=0, n_addr=1 << 20, per_thread=64):
+    """Device-scope FP64 and uint64 atomic-add rates of this GPU on 1M
+    distinct counters (molflow_sources_amd/libmfg_microbench.so, a
+    measurement helper): the peak the trace kernel's tally atomics are
+    reported against."""
+    import ctypes
+    path = os.path.join(ROOT, "molflow_sources_amd", "libmfg_microbench.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.mfgb_atomic_rates.restype = ctypes.c_int
+    lib.mfgb_atomic_rates.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    f, u = ctypes.c_double(), ctypes.c_double()
+    if lib.mfgb_atomic_rates(device, n_addr, per_thread, ctypes.byref(f), ctypes.byref(u)) != 0:
+        return None
+    return {"f64_per_s": f.value, "u64_per_s": u.value, "counters": n_addr,
+            "ops": "one FP64 (or uint64) atomic add without return per lane, 8 blocks of 256 per CU"}
+
+
+def atomics_from_profile():
+    """Global atomics of one steady-state trace launch, from the committed
+    rocprofv3 TCC_ATOMIC summary (scaled by the count it reads for the
+    microbenchmark's known number of atomics)."""
+    p = os.path.join(ROOT, "profiles", "trace_beam_pipe_atomics.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("atomics_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +287,14 @@ def main():
                      "kernel_ms_per_launch": launch_s * 1e3},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(model, name="tube" if tube else "beam-pipe"),
     }
+    peak = atomic_peak()
+    atom = None if tube else atomics_from_profile()
+    if peak:
+        out["atomics"] = {"bound": "l2_atomics", "achieved": atom / launch_s if atom else None,
+                          "peak": peak["f64_per_s"], "unit": "atomic ops/s",
+                          "frac": atom / launch_s / peak["f64_per_s"] if atom else None,
+                          "atomics_per_hit": atom / hits_per_launch if atom else None,
+                          "peak_u64": peak["u64_per_s"], "peak_measured_by": peak["ops"]}
     if tube:
         # facet 1 is the outlet disk: transmission = absorbed there / desorbed
         out["config"]["transmission"] = float(t.facets["nb_abs_equiv"][0][1]) / float(t.glob[0]["nb_desorbed"])

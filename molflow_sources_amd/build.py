@@ -36,6 +36,21 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+MICROBENCH = os.path.join(HERE, "libmfg_microbench.so")
+
+
+def build_microbench(verbose: bool = False) -> str:
+    """The atomic-throughput measurement helper (include/mfg_microbench.h)."""
+    src = os.path.join(CSRC, "mfg_microbench.hip")
+    tmp = MICROBENCH + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-x", "hip", "-shared", src, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, MICROBENCH)
+    return MICROBENCH
+
+
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "run_model.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "run_model")
@@ -56,3 +71,4 @@ def build_driver(verbose: bool = False) -> str:
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_driver(verbose=True))
+    print(build_microbench(verbose=True))

@@ -122,3 +122,19 @@ def test_python_moment_parser_matches_oracle(oracle_lib):
     from oracle import mfcpu
     for c, w in (("1e-3,1e-3,4e-2", 1e-3), ("0.25e-3,0.5e-3,49.8e-3", 5e-4), ("0", 1.0)):
         assert np.array_equal(np.array(parse_user_moment(c, w)), mfcpu.parse_user_moment(c, w))
+
+
+def test_microbench_library_exports_its_header():
+    """The atomic-throughput helper (measurement only) loads and exports the
+    symbol its header declares; no GPU call is made."""
+    import ctypes
+    import re
+    lib_path = os.path.join(ROOT, "molflow_sources_amd", "libmfg_microbench.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("libmfg_microbench.so not built")
+    txt = open(os.path.join(ROOT, "include", "mfg_microbench.h")).read()
+    names = re.findall(r"^int\s+(mfgb_\w+)\(", txt, flags=re.M)
+    assert names == ["mfgb_atomic_rates"]
+    lib = ctypes.CDLL(lib_path)
+    for n in names:
+        assert hasattr(lib, n)
