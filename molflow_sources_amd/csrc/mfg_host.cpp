@@ -372,6 +372,7 @@ struct Collapse {
                 for (int i = 0; i < cn.count; ++i) leaf_ids.push_back(prims[cn.first + i]);
                 nd.child[c] = ~first;
                 nd.count[c] = (uint8_t)cn.count;
+                nd.leaf_mask |= (uint8_t)(1u << c);
             } else {
                 nd.child[c] = build(ch[c], depth + 1) + node_base;
                 nd.count[c] = 0;

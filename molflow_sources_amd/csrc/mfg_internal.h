@@ -129,7 +129,8 @@ struct alignas(128) BvhNode {
     uint8_t count[8];           // facets in a leaf child (0 for inner/empty)
     uint8_t qlo[3][8];          // [axis][child]
     uint8_t qhi[3][8];
-    uint8_t pad[24];
+    uint8_t leaf_mask;          // bit c set when child slot c is a leaf
+    uint8_t pad[23];
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 bytes");
 

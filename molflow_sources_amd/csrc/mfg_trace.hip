@@ -301,7 +301,7 @@ __device__ __forceinline__ Ray32 make_ray32(d3 P, d3 D) {
 // A node's 128 B (112 used) as seven 16-byte loads issued together (one
 // round trip). Words: c0 = origin x,y,z + (ex, ey, ez, valid) bytes;
 // c1, c2 = child words 0-3, 4-7; c3 = leaf counts (x,y) + qlo x (z,w);
-// c4 = qlo y, qlo z; c5 = qhi x, qhi y; c6 = qhi z (x,y).
+// c4 = qlo y, qlo z; c5 = qhi x, qhi y; c6 = qhi z (x,y), leaf mask (z byte 0).
 struct NodeRegs {
     uint4 c0, c1, c2, c3, c4, c5, c6;
 };
@@ -580,12 +580,9 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         uint32_t m = node_hits(nd, r, tbf, tn);
         // leaf children first (one loop for all slots, so lanes whose leaves
         // sit in different child slots test their facets together)
-        uint32_t lm = 0, im = 0;
-#pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            if (!(m & (1u << c))) continue;
-            if (node_child(nd, c) < 0) lm |= 1u << c; else im |= 1u << c;
-        }
+        const uint32_t leaves = nd.c6.z & 0xffu;   // BvhNode::leaf_mask
+        uint32_t lm = m & leaves;
+        const uint32_t im = m & ~leaves;
         while (lm) {
             uint32_t c = (uint32_t)__builtin_ctz(lm);
             lm &= lm - 1u;
