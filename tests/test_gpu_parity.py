@@ -76,6 +76,22 @@ def test_first_hit_bit_exact_4_wide(gpu):
     assert np.all(np.abs(tg[hit] - tc[hit]) <= 1e-5 * tc[hit])
 
 
+@pytest.mark.parametrize("width,leaf", [(8, 1), (4, 1)])
+def test_first_hit_bit_exact_one_facet_leaves(gpu, width, leaf):
+    """Leaves of one facet (the shape the autotuner picks for the tube) on
+    the tube and on the polygon set: the same first hits as brute force."""
+    for m, lo, hi in ((geometry.tube_model(), [-1.2, -1.2, -0.5], [1.2, 1.2, 10.5]),
+                      (polygon_zoo_model(), [-1.3, -1.3, -0.5], [1.3, 1.3, 3.5])):
+        o, d = random_rays(30_000, lo, hi, seed=31)
+        gpu.load(m)
+        gpu.rebuild_accel(width, leaf)
+        fg, tg, _, _ = gpu.intersect(o, d)
+        fc, tc, _, _ = CpuModel(m).intersect(o, d, brute=True)
+        assert np.array_equal(fg, fc), f"{m.name}: {int((fg != fc).sum())} first-hit facets differ"
+        hit = fc >= 0
+        assert np.all(np.abs(tg[hit] - tc[hit]) <= 1e-5 * tc[hit])
+
+
 def test_first_hit_bit_exact_large_polygons(gpu):
     """Polygons of more than 4 vertices (non-convex star, comb with equal
     vertex heights and horizontal edges, 64-gon, C shape) go through the
