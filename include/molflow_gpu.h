@@ -314,7 +314,8 @@ int mfg_request_stop(mfg_context* ctx);
 /* Wall-clock limit of one trace launch (default 60 s, 0 < seconds <= 86400).
  * A wave that reaches it starts no particle, parks its particles in flight
  * (carry enabled) or ends them where they are (carry disabled), and the run
- * returns with stopped = 2. It bounds every launch, also one whose particles
+ * returns with stopped = 2. Parked particles the launch never resumed stay
+ * parked. It bounds every launch, also one whose particles
  * can never be absorbed and whose max_events is 0. No counterpart in the
  * reference, whose SimulationMCStep returns after a given number of events
  * (Particle.cpp:207-396) instead. */
@@ -328,8 +329,31 @@ int mfg_get_facet_offsets(const mfg_context* ctx, int kind, int64_t* out, size_t
 
 /* Add the device tallies into caller buffers (UpdateMCHits + GlobalSimuState
  * +=, Particle.cpp:21-73, MolflowSimGeom.cpp:452-458). Device tallies are
- * not cleared (call mfg_reset for that). */
+ * not cleared (mfg_take_tallies clears them, mfg_reset restarts the run). */
 int mfg_read_tallies(mfg_context* ctx, const mfg_tally_view* out);
+
+/* Periodic report: add the device tallies into caller buffers, then zero
+ * them, keeping the particles parked between runs, the particle counter
+ * and the random streams. The UpdateHitsAndLog role (Particle.cpp:1294-1312:
+ * globalState += tmpState, then tmpState->Reset()), so k runs with a take
+ * after each add up to one uninterrupted run. The log and leak buffers are
+ * copied (not added) and restart empty. One model only (MFG_ERR_STATE with
+ * variants: read each variant, then call mfg_clear_tallies). */
+int mfg_take_tallies(mfg_context* ctx, const mfg_tally_view* out);
+/* Zero the device tallies of every loaded variant, keeping parked
+ * particles and streams (tmpState->Reset(), Particle.cpp:1309). */
+int mfg_clear_tallies(mfg_context* ctx);
+/* Empty the particle log (ReinitializeParticleLog,
+ * MolflowSimulation.cpp:15-35). */
+int mfg_reinit_particle_log(mfg_context* ctx);
+
+/* Check a model view without loading it (SanityCheckModel,
+ * MolflowSimulation.cpp:57-71, MolflowSimGeom.cpp:337-404): runs the same
+ * preparation as mfg_load_model plus the reference's own checks, writes one
+ * line per problem into out (cap bytes, NUL-terminated, may be NULL) and
+ * their count into *n_problems. Needs no device. Returns MFG_OK whenever
+ * the check ran; n_problems == 0 means the model loads. */
+int mfg_sanity_check(const mfg_model_view* model, char* out, size_t cap, uint32_t* n_problems);
 
 /* Bytes of one full tally state (GetHitsSize, MolflowSimulation.cpp:156-160). */
 int mfg_hits_size(const mfg_context* ctx, uint64_t* bytes);
@@ -424,7 +448,10 @@ int mfg_enable_graph(mfg_context* ctx, int enable);
  * Requires low-flux mode (the path does not depend on sticking). During
  * later mfg_run calls the forward pass accumulates, for every target t,
  * J0[t] = sum of hit weights (nbHitEquiv) and the path-score Jacobian
- * J[t][g] = d J0[t] / d s_g in FP64. */
+ * J[t][g] = d J0[t] / d s_g in FP64. Each group starts from the constant
+ * sticking its facets share (MFG_ERR_INVALID if they differ or one has a
+ * time-dependent sticking); if that value is not inside (0,1), runs are
+ * refused (MFG_ERR_STATE) until mfg_adjoint_set_sticking sets one. */
 int mfg_adjoint_setup(mfg_context* ctx, const int32_t* group_of_facet, uint32_t n_groups,
                       const int32_t* target_facets, uint32_t n_targets);
 /* Set the sticking value of each group (overrides facet sticking). */

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "molflow_gpu.h"
 
@@ -85,6 +86,35 @@ public:
     /* UpdateMCHits (Particle.cpp:21-73): adds the device tallies into the
      * caller's buffers (laid out as GlobalSimuState, see molflow_gpu.h) */
     void UpdateMCHits(const mfg_tally_view& view) const { check(mfg_read_tallies(ctx_, &view), "mfg_read_tallies"); }
+
+    /* UpdateHitsAndLog (Particle.cpp:1294-1312): adds the device tallies
+     * into the caller's buffers, then clears them on the device (the
+     * tmpState->Reset() step); particles in flight and the particle stream
+     * are kept, so periodic reports add up to one uninterrupted run */
+    void UpdateHitsAndLog(const mfg_tally_view& view) { check(mfg_take_tallies(ctx_, &view), "mfg_take_tallies"); }
+
+    /* ReinitializeParticleLog (MolflowSimulation.h:35) */
+    int ReinitializeParticleLog() {
+        check(mfg_reinit_particle_log(ctx_), "mfg_reinit_particle_log");
+        return 0;
+    }
+
+    /* SanityCheckModel (MolflowSimulation.h:29): problems that stop the
+     * model from loading plus the reference's own warnings; no device
+     * needed */
+    static std::vector<std::string> SanityCheckModel(const mfg_model_view& view) {
+        std::vector<char> buf(1 << 16);
+        uint32_t n = 0;
+        if (mfg_sanity_check(&view, buf.data(), buf.size(), &n) != MFG_OK) return {"mfg_sanity_check failed"};
+        std::vector<std::string> out;
+        std::string cur;
+        for (const char* p = buf.data(); *p; ++p) {
+            if (*p == '\n') { out.push_back(cur); cur.clear(); }
+            else cur += *p;
+        }
+        if (!cur.empty()) out.push_back(cur);
+        return out;
+    }
 
     mfg_tally_layout Layout() const {
         mfg_tally_layout L{};

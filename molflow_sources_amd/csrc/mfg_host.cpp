@@ -637,7 +637,14 @@ void prepare(HostModel& M, const mfg_model_view* v) {
             if (d.super_dest - 1 >= (int)M.n_struct || d.super_dest < 0) throw Err(MFG_ERR_INVALID, "invalid structure link on facet " + std::to_string(f));
             fl |= FF_LINK; feat |= FEAT_TELEPORT;
         }
-        if (d.super_idx >= (int)M.n_struct) throw Err(MFG_ERR_INVALID, "facet structure index out of range");
+        // structure -1 means "in every structure"; anything else must name one
+        if (d.super_idx < -1 || d.super_idx >= (int)M.n_struct)
+            throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": structure index " + std::to_string(d.super_idx) +
+                                           " out of range [-1, " + std::to_string(M.n_struct) + ")");
+        // teleport destination + 1, 0 none, -1 back to the previous destination
+        if (d.teleport_dest < -1 || d.teleport_dest > (int32_t)nF)
+            throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + ": teleport destination " +
+                                           std::to_string(d.teleport_dest) + " out of range [-1, " + std::to_string(nF) + "]");
         if (d.diffuse_part > 0.999999) fl |= FF_DIFFUSE;
         if (d.accomodation > 0.9999) fl |= FF_FULL_ACC;
         if (d.count_des) fl |= FF_COUNT_DES;
@@ -864,6 +871,7 @@ struct mfg_context {
     DevBuf<uint32_t> adj_counts;
     std::vector<int32_t> adj_group_of, adj_target_list;
     uint64_t adj_des = 0;
+    bool adj_s_ok = false;               // every group sticking inside (0,1)
     // graph
     bool graph_on = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -1076,17 +1084,27 @@ void upload_models(mfg_context* c) {
     HIPCHK(hipStreamSynchronize(s));
 }
 
-void zero_tallies(mfg_context* c) {
+// Zero the device tallies. keep_particles = true is the tmpState->Reset()
+// of a periodic report (Particle.cpp:1294-1312): the parked particles, the
+// carry control words and the particle streams stay as they are. Otherwise
+// (ResetSimulation) the parked particles are dropped too.
+void zero_tallies(mfg_context* c, bool keep_particles = false) {
     hipStream_t s = c->stream;
     auto z = [&](void* p, size_t b) { if (mfg_launch_zero(p, b, s)) throw Err(MFG_ERR_DEVICE, "zero kernel launch failed"); };
     z(c->fc.p, c->fc.bytes()); z(c->force.p, c->force.bytes()); z(c->tex.p, c->tex.bytes());
     z(c->dir.p, c->dir.bytes()); z(c->prof.p, c->prof.bytes()); z(c->am.p, c->am.bytes());
     z(c->gh.p, c->gh.bytes()); z(c->fh.p, c->fh.bytes()); z(c->glob.p, c->glob.bytes());
-    z(c->counters.p, c->counters.bytes());
+    if (keep_particles) {
+        // log and leak counts, time-limit flag; not the carry words (4..6)
+        z(c->counters.p + 1, 3 * sizeof(unsigned long long));
+        z(c->counters.p + 7, sizeof(unsigned long long));
+    } else {
+        z(c->counters.p, c->counters.bytes());
+    }
     if (c->adj_j0.p) { z(c->adj_j0.p, c->adj_j0.bytes()); z(c->adj_jac.p, c->adj_jac.bytes()); }
     HIPCHK(hipStreamSynchronize(s));
     c->adj_des = 0;
-    c->in_flight = 0;   // the counters above include the carry control words
+    if (!keep_particles) c->in_flight = 0;   // the counters above include the carry control words
 }
 
 // Renumber a variant's nodes breadth-first from its roots, so the top of
@@ -1308,6 +1326,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     launch_shape(c, feats, grid, block, waves);
     if (c->adj_groups) {
         if (c->carry_on || c->in_flight) throw Err(MFG_ERR_STATE, "the adjoint traces every particle to its end: disable carry and drain first");
+        if (!c->adj_s_ok)
+            throw Err(MFG_ERR_STATE, "a sticking group's value is not inside (0,1): set it with mfg_adjoint_set_sticking");
         size_t need = (size_t)grid * block * c->adj_groups;
         if (c->adj_counts.n < need) c->adj_counts.alloc(need);
     }
@@ -1338,7 +1358,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
             throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
         if (events) HIPCHK(hipEventRecord(c->ev0, s));
         if (mfg_launch_trace(p, feats, waves, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
-        if (p.carry && mfg_launch_carry_flip(p.carry_ctl, s)) throw Err(MFG_ERR_DEVICE, "carry flip launch failed");
+        if (p.carry && mfg_launch_carry_flip(p.carry_ctl, p.next_particle, p.carry, p.carry_cap, c->counters.p + 7, s))
+            throw Err(MFG_ERR_DEVICE, "carry flip launch failed");
     };
     if (c->graph_on) {
         HIPCHK(hipEventRecord(c->ev0, s));
@@ -1363,9 +1384,10 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     unsigned long long expired = 0;
     HIPCHK(hipMemcpy(&expired, c->counters.p + 3, sizeof(expired), hipMemcpyDeviceToHost));
     if (p.carry) {
-        unsigned long long ctl[3];
+        unsigned long long ctl[4];   // carry control words and the overflow flag
         HIPCHK(hipMemcpy(ctl, c->counters.p + 4, sizeof(ctl), hipMemcpyDeviceToHost));
         c->in_flight = ctl[ctl[2] & 1ull];
+        if (ctl[3]) throw Err(MFG_ERR_STATE, "carry buffer overflow: parked particles were lost");
     }
     uint64_t hits = 0, des = 0;
     for (size_t v = 0; v < nv; ++v) {
@@ -1639,6 +1661,72 @@ int mfg_read_variant_tallies(mfg_context* c, uint32_t vi, const mfg_tally_view* 
     return guarded(c, [&] { if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded"); add_tallies(c, vi, o); });
 }
 
+int mfg_clear_tallies(mfg_context* c) {
+    return guarded(c, [&] {
+        if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
+        zero_tallies(c, true);
+    });
+}
+
+int mfg_take_tallies(mfg_context* c, const mfg_tally_view* o) {
+    return guarded(c, [&] {
+        if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
+        if (c->dvars.size() != 1)
+            throw Err(MFG_ERR_STATE, "several variants loaded: read each with mfg_read_variant_tallies, then mfg_clear_tallies");
+        add_tallies(c, 0, o);
+        zero_tallies(c, true);
+    });
+}
+
+int mfg_reinit_particle_log(mfg_context* c) {
+    return guarded(c, [&] {
+        if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
+        if (mfg_launch_zero(c->counters.p + 1, sizeof(unsigned long long), c->stream))
+            throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mfg_sanity_check(const mfg_model_view* v, char* out, size_t cap, uint32_t* n_problems) {
+    if (!n_problems) return MFG_ERR_INVALID;
+    std::vector<std::string> log;
+    try {
+        HostModel M;
+        prepare(M, v);
+    } catch (const std::exception& e) {
+        log.push_back(e.what());
+    }
+    // the reference's own checks (MolflowSimulationModel::SanityCheck,
+    // MolflowSimGeom.cpp:337-404), on what the view holds
+    if (v && v->struct_size == sizeof(mfg_model_view) && v->facets && v->settings) {
+        bool any_source = false;
+        for (uint32_t f = 0; f < v->n_facets; ++f) {
+            const mfg_facet_desc& d = v->facets[f];
+            const bool counters = d.count_des || d.count_abs || d.count_refl || d.count_trans || d.count_dir;
+            if (!d.is_textured && d.tex_width_precise * d.tex_height_precise > 0)
+                log.push_back("[Facet #" + std::to_string(f + 1) + "] Untextured facet with texture size");
+            else if (!d.is_textured && counters)
+                log.push_back("[Facet #" + std::to_string(f + 1) + "] Untextured facet with texture counters");
+            if (d.desorb_type != MFG_DES_NONE && d.temperature_param >= 0)
+                log.push_back("[Facet " + std::to_string(f + 1) +
+                              "]: Time-dependent temperature not allowed on facets with outgassing");
+            any_source = any_source || d.desorb_type != MFG_DES_NONE;
+        }
+        if (v->settings->enable_decay && v->settings->half_life <= 0.0)
+            log.push_back("Particle decay is set, but half life was not set");
+        if (!any_source) log.push_back("No desorption facet found");
+    }
+    std::string all;
+    for (auto& s : log) all += s + "\n";
+    if (out && cap) {
+        size_t n = std::min(cap - 1, all.size());
+        std::memcpy(out, all.data(), n);
+        out[n] = '\0';
+    }
+    *n_problems = (uint32_t)log.size();
+    return MFG_OK;
+}
+
 int mfg_hits_size(const mfg_context* c, uint64_t* bytes) {
     return guarded(const_cast<mfg_context*>(c), [&] {
         if (!bytes) throw Err(MFG_ERR_INVALID, "null output");
@@ -1878,9 +1966,26 @@ int mfg_adjoint_setup(mfg_context* c, const int32_t* group_of_facet, uint32_t G,
         c->adj_groups = G; c->adj_targets = T;
         c->adj_j0.alloc(T); c->adj_jac.alloc((size_t)T * G); c->adj_s.alloc(G);
         c->adj_up.alloc(T); c->adj_grad.alloc(G);
-        std::vector<double> s0(G);
-        for (uint32_t g = 0; g < G; ++g) s0[g] = 0.0;
-        for (uint32_t f = 0; f < nF; ++f) if (group_of_facet[f] >= 0) s0[group_of_facet[f]] = M.phys[f].sticking;
+        // each group starts from its facets' common constant sticking; a
+        // group whose facets disagree is refused, and one whose value is not
+        // inside (0,1) (where d/ds = -n/(1-s) + 1/s is defined) must be set
+        // with mfg_adjoint_set_sticking before the next run
+        std::vector<double> s0(G, 0.0);
+        std::vector<char> seen(G, 0);
+        for (uint32_t f = 0; f < nF; ++f) {
+            const int32_t g = group_of_facet[f];
+            if (g < 0) continue;
+            if (M.phys[f].sticking_param >= 0)
+                throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + " of sticking group " + std::to_string(g) +
+                                               " has a time-dependent sticking");
+            if (seen[g] && s0[g] != M.phys[f].sticking)
+                throw Err(MFG_ERR_INVALID, "sticking group " + std::to_string(g) + " mixes facets of different sticking");
+            s0[g] = M.phys[f].sticking;
+            seen[g] = 1;
+        }
+        c->adj_s_ok = true;
+        for (uint32_t g = 0; g < G; ++g)
+            if (!(s0[g] > 0.0 && s0[g] < 1.0)) c->adj_s_ok = false;
         HIPCHK(hipMemcpyAsync(c->adj_s.p, s0.data(), G * sizeof(double), hipMemcpyHostToDevice, c->stream));
         c->features |= FEAT_ADJOINT;
         zero_tallies(c);
@@ -1894,6 +1999,7 @@ int mfg_adjoint_set_sticking(mfg_context* c, const double* s, uint32_t G) {
         for (uint32_t g = 0; g < G; ++g)
             if (!(s[g] > 0.0 && s[g] < 1.0)) throw Err(MFG_ERR_INVALID, "group sticking must be in (0,1)");
         HIPCHK(hipMemcpy(c->adj_s.p, s, G * sizeof(double), hipMemcpyHostToDevice));
+        c->adj_s_ok = true;
     });
 }
 

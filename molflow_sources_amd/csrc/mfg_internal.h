@@ -326,7 +326,10 @@ struct IsectParams {
 // kernel launchers (mfg_trace.hip)
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream);
 int mfg_launch_intersect(const IsectParams& p, void* stream);
-int mfg_launch_carry_flip(unsigned long long* carry_ctl, void* stream);
+// after a carry launch: append the parked particles the launch did not
+// claim (time limit) to the output half, then swap the halves
+int mfg_launch_carry_flip(unsigned long long* carry_ctl, const unsigned long long* claimed, CarryRec* carry,
+                          uint64_t cap, unsigned long long* overflow, void* stream);
 int mfg_launch_zero(void* ptr, uint64_t bytes, void* stream);
 int mfg_launch_backward(const double* jac, const double* up, double* grad, uint32_t n_targets, uint32_t n_groups,
                         double scale, void* stream);

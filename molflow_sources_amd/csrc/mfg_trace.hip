@@ -1289,7 +1289,11 @@ struct Tracer {
         last_hit = f;
         trace(f, 1);
     }
-    // PerformTeleport (Particle.cpp:76-183)
+    // PerformTeleport (Particle.cpp:76-183). "Back to origin" (teleport_dest
+    // = -1) goes to the destination of the particle's previous teleport:
+    // the reference records teleportedFrom = destIndex (:122). A destination
+    // in every structure (super_idx -1) keeps the particle's structure; the
+    // reference would index its structure list with -1 there (:121, :245).
     __device__ void teleport(int32_t f) {
         const DFacet* F = P.facets + f;
         const DPhys& ph = P.phys[f];
@@ -1300,7 +1304,7 @@ struct Tracer {
         const DFacet* D = P.facets + dg;
         const DPhys& dph = P.phys[dg];
         if (dph.super_idx >= 0) structure = dph.super_idx;
-        tele_from = local(f);
+        tele_from = dest;
         int m = moment(time);
         if ((ph.flags & FF_COUNT_TRANS) && ph.tex_off >= 0) texture(F, ph, m, true, 2.0, 2.0);
         if (ph.flags & FF_COUNT_DIR) direction(ph, m);
@@ -1731,11 +1735,40 @@ __global__ void __launch_bounds__(256, WAVES) k_trace(TraceParams P) {
     trace_body<FEAT>(P);
 }
 
-// After a launch: the half it resumed from is consumed, and the half that
-// received its leftovers becomes the next launch's input.
-__global__ void k_carry_flip(unsigned long long* ctl) {
+// Parked particles that a launch did not claim: the queue hands out items
+// [0, claimed) and the first n_in of them are the resumed particles, so
+// [min(claimed, n_in), n_in) were never taken. That happens only when the
+// launch time limit stops the claims (a dry queue means every item was
+// claimed). These records are appended to the output half after the ones
+// the waves parked themselves.
+__device__ __forceinline__ void carry_counts(const unsigned long long* ctl, const unsigned long long* claimed,
+                                             uint64_t& half, uint64_t& first, uint64_t& n, uint64_t& n_out) {
+    half = ctl[2] & 1ull;
+    const uint64_t n_in = ctl[half];
+    first = *claimed < n_in ? *claimed : n_in;
+    n = n_in - first;
+    n_out = ctl[half ^ 1ull];
+}
+__global__ void k_carry_rescue(const unsigned long long* ctl, const unsigned long long* claimed, CarryRec* carry,
+                               uint64_t cap) {
+    uint64_t half, first, n, n_out;
+    carry_counts(ctl, claimed, half, first, n, n_out);
+    const CarryRec* in = carry + half * cap;
+    CarryRec* out = carry + (half ^ 1ull) * cap;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (n_out + i < cap) out[n_out + i] = in[first + i];
+}
+// After a launch (and the rescue): the half it resumed from is consumed,
+// and the half that received its leftovers becomes the next launch's
+// input. overflow is set if the rescued records did not fit.
+__global__ void k_carry_flip(unsigned long long* ctl, const unsigned long long* claimed, uint64_t cap,
+                             unsigned long long* overflow) {
     if (threadIdx.x == 0) {
-        unsigned long long half = ctl[2] & 1ull;
+        uint64_t half, first, n, n_out;
+        carry_counts(ctl, claimed, half, first, n, n_out);
+        uint64_t total = n_out + n;
+        if (total > cap) { *overflow = 1ull; total = cap; }
+        ctl[half ^ 1ull] = total;
         ctl[half] = 0ull;
         ctl[2] = half ^ 1ull;
     }
@@ -1840,8 +1873,15 @@ int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int gri
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mfg_launch_carry_flip(unsigned long long* ctl, void* stream) {
-    hipLaunchKernelGGL(k_carry_flip, dim3(1), dim3(64), 0, (hipStream_t)stream, ctl);
+int mfg_launch_carry_flip(unsigned long long* ctl, const unsigned long long* claimed, CarryRec* carry, uint64_t cap,
+                          unsigned long long* overflow, void* stream) {
+    uint64_t blocks = (cap + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_carry_rescue, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, ctl, claimed, carry,
+                       cap);
+    if (hipGetLastError() != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_carry_flip, dim3(1), dim3(64), 0, (hipStream_t)stream, ctl, claimed, cap, overflow);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

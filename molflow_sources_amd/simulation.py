@@ -7,6 +7,8 @@
     sim.reset(seed=42)                    # ResetSimulation
     res = sim.run(n_des)                  # desorb n_des particles to the end
     t = sim.read()                        # UpdateMCHits into a fresh state
+    sim.take(t)                           # UpdateHitsAndLog: add, then clear
+    problems = sanity_check(model)        # SanityCheckModel (no device needed)
 
 Every method raises MolflowGpuError with the library's message on failure;
 there is no CPU fallback.
@@ -174,6 +176,21 @@ class GpuSimulation:
             self._check(self._lib.mfg_read_variant_tallies(self.h, variant, C.byref(v)), "mfg_read_variant_tallies")
         return tallies
 
+    def take(self, tallies: Tallies | None = None) -> Tallies:
+        """Add the device tallies into `tallies` and clear them on the device,
+        keeping parked particles (a periodic report, UpdateHitsAndLog)."""
+        if tallies is None:
+            tallies = Tallies(self.layout(), self.offsets())
+        v = tallies.view()
+        self._check(self._lib.mfg_take_tallies(self.h, C.byref(v)), "mfg_take_tallies")
+        return tallies
+
+    def clear_tallies(self) -> None:
+        self._check(self._lib.mfg_clear_tallies(self.h), "mfg_clear_tallies")
+
+    def reinit_particle_log(self) -> None:
+        self._check(self._lib.mfg_reinit_particle_log(self.h), "mfg_reinit_particle_log")
+
     def hits_size(self) -> int:
         b = C.c_uint64()
         self._check(self._lib.mfg_hits_size(self.h, C.byref(b)), "mfg_hits_size")
@@ -226,6 +243,19 @@ class GpuSimulation:
         g = np.zeros(ng)
         self._check(self._lib.mfg_adjoint_backward(self.h, _dp(up), _dp(g)), "mfg_adjoint_backward")
         return g
+
+
+def sanity_check(model) -> list[str]:
+    """SanityCheckModel (reference src/Simulation/MolflowSimulation.cpp:57-71):
+    the problems that would stop `model` from loading, plus the reference's
+    own warnings; an empty list means the model loads. Needs no device."""
+    v = model.view()
+    buf = C.create_string_buffer(65536)
+    n = C.c_uint32()
+    rc = lib().mfg_sanity_check(C.byref(v), buf, len(buf), C.byref(n))
+    if rc:
+        raise MolflowGpuError(f"mfg_sanity_check failed ({rc})")
+    return [line for line in buf.value.decode().split("\n") if line][: n.value]
 
 
 def maxwell_cdf(gas_mass: float):

@@ -833,7 +833,9 @@ void load_model(Model& M, const mfg_model_view* v) {
     M.struct_facets.assign(M.n_struct, {});
     for (uint32_t f = 0; f < v->n_facets; ++f) {
         int si = M.F[f].d.super_idx;
-        if (si >= M.n_struct) throw std::runtime_error("facet structure index out of range");
+        if (si < -1 || si >= M.n_struct) throw std::runtime_error("facet structure index out of range");
+        int td = M.F[f].d.teleport_dest;
+        if (td < -1 || td > (int)v->n_facets) throw std::runtime_error("teleport destination out of range");
         if (si < 0) { for (auto& sf : M.struct_facets) sf.push_back((int)f); }
         else M.struct_facets[si].push_back((int)f);
         int sd = M.F[f].d.super_dest;
@@ -1327,14 +1329,22 @@ struct Tracer {
         last_hit = fid;
         trace(fid, 1);
     }
-    void perform_teleport(int fid) {  // PerformTeleport
+    // PerformTeleport (Particle.cpp:76-183). teleported_from becomes the
+    // destination, as in the reference's live code (:122); "back to origin"
+    // (teleport_dest -1) therefore returns to the last teleport destination.
+    // The reference's retry loop for a point outside the destination polygon
+    // (:146-158) increments its try counter after the loop, so it only ends
+    // on success; here it ends after 1000 tries. A destination in every
+    // structure (super_idx -1) keeps the current structure; the reference
+    // would set ray.structure = -1 (:121) and fail at its next Intersect.
+    void perform_teleport(int fid) {
         const Facet& F = M.F[fid];
         int dest = F.d.teleport_dest == -1 ? teleported_from : F.d.teleport_dest - 1;
         trace(fid, 4);
         if (dest < 0 || dest >= (int)M.F.size()) { last_hit = fid; return; }
         const Facet& D = M.F[dest];
         if (D.d.super_idx >= 0) structure = D.d.super_idx;
-        teleported_from = fid;
+        teleported_from = dest;
         int m = moment_index(M, time);
         if (F.d.count_trans && F.tex_off >= 0) texture(F, m, true, 2.0, 2.0);
         if (F.d.count_dir) direction(F, m);
@@ -1617,11 +1627,45 @@ double mfcpu_total_desorbed_molecules(const mfcpu_model* m) { return m->M.total_
 double mfcpu_final_outgassing_rate(const mfcpu_model* m) { return m->M.final_rate; }
 double mfcpu_latest_moment(const mfcpu_model* m) { return m->M.latest; }
 
+// One PerformScatter (Particle.cpp:840-886, Physics.cpp:41-63) of n
+// particles moving along +z at speed 1 with mass ratio and rho as given;
+// returns the new direction's z component and the new speed of each. A
+// test hook for the scattering laws; draws use particle i's stream.
+int mfcpu_scatter_sample(double mass_ratio, double rho, uint64_t n, uint64_t seed, double* out_cos,
+                         double* out_speed) {
+    if (!out_cos || !out_speed || !(rho > 0)) return MFG_ERR_INVALID;
+    Model M{};
+    std::memset(&M.s, 0, sizeof(M.s));
+    M.s.scatter_enabled = 1;
+    M.s.scatter_mass_ratio = mass_ratio;
+    M.s.scatter_rho = rho;
+    M.s.scatter_mean_free_path_cm = 1.0;
+    Tally T;
+    Tracer tr(M, T);
+    for (uint64_t i = 0; i < n; ++i) {
+        tr.rng.init(seed, i);
+        tr.dir = {0.0, 0.0, 1.0};
+        tr.velocity = 1.0;
+        tr.dist = 0.0;
+        tr.perform_scatter();
+        out_cos[i] = tr.dir.z;
+        out_speed[i] = tr.velocity;
+    }
+    return 0;
+}
+
 int mfcpu_adjoint_run(mfcpu_model* mm, uint64_t seed, uint64_t first, uint64_t n_des, const int32_t* group,
                       uint32_t G, const double* gs, const int32_t* targets, uint32_t nT, int threads,
                       double* j0, double* jac, uint64_t max_events) {
     if (!mm || !mm->M.s.low_flux_mode) return MFG_ERR_INVALID;
     const Model& M = mm->M;
+    // the path score d/ds = -n/(1-s) (+ 1/s on the absorbed part) needs every
+    // group sticking strictly inside (0,1)
+    if (!group || !gs || !targets || G == 0 || nT == 0) return MFG_ERR_INVALID;
+    for (uint32_t g = 0; g < G; ++g)
+        if (!(gs[g] > 0.0 && gs[g] < 1.0)) return MFG_ERR_INVALID;
+    for (uint32_t t = 0; t < nT; ++t)
+        if (targets[t] < 0 || (size_t)targets[t] >= M.F.size()) return MFG_ERR_INVALID;
     int nt = 1;
 #ifdef _OPENMP
     nt = threads > 0 ? threads : omp_get_max_threads();

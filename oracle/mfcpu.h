@@ -64,6 +64,12 @@ double mfcpu_total_desorbed_molecules(const mfcpu_model* m);
 double mfcpu_final_outgassing_rate(const mfcpu_model* m);
 double mfcpu_latest_moment(const mfcpu_model* m);
 
+/* Test hook of the hard-sphere scattering law: one scatter of each of n
+ * particles moving along +z at unit speed; writes the new direction's z
+ * component and the new speed. */
+int mfcpu_scatter_sample(double mass_ratio, double rho, uint64_t n, uint64_t seed, double* out_cos,
+                         double* out_speed);
+
 /* Sticking adjoint (low-flux mode): J0[t] and J[t][g] accumulated over
  * particles first..first+n_des-1, with group sticking values overriding
  * the facet sticking of grouped facets. */

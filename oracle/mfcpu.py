@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
         for n in ("mfcpu_total_desorbed_molecules", "mfcpu_final_outgassing_rate", "mfcpu_latest_moment"):
             getattr(L, n).restype = C.c_double
             getattr(L, n).argtypes = [vp]
+        L.mfcpu_scatter_sample.argtypes = [C.c_double, C.c_double, u64, u64, dp, dp]
         L.mfcpu_adjoint_run.argtypes = [vp, u64, u64, u64, C.POINTER(i32), C.c_uint32, dp, C.POINTER(i32),
                                         C.c_uint32, C.c_int, dp, dp, u64]
         _lib = L
@@ -184,6 +185,15 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().mfcpu_philox(c, k, o)
     return list(o)
+
+
+def scatter_sample(mass_ratio: float, rho: float, n: int, seed: int = 1):
+    """(cos of the deflection, new speed) of n unit-speed scatters."""
+    c, v = np.zeros(n), np.zeros(n)
+    rc = lib().mfcpu_scatter_sample(mass_ratio, rho, n, seed, _dp(c), _dp(v))
+    if rc:
+        raise RuntimeError("mfcpu_scatter_sample failed")
+    return c, v
 
 
 def maxwell_cdf(gas_mass: float):

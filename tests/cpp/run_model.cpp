@@ -1,7 +1,9 @@
 // run_model.cpp -- C++ host driver over include/molflow_gpu.hpp, used by the
 // GPU tests to exercise the boundary from C++ the way a C++ simulation
-// manager would: load a model, build the BVH, reset, run steps (optionally
-// keeping particles in flight between them), drain, read the tallies.
+// manager would: check and load a model, build the BVH, reset, run steps
+// (optionally keeping particles in flight between them) with a periodic
+// report after each (UpdateHitsAndLog: add into the host state, clear the
+// device tallies), drain, and report once more.
 //
 //   run_model MODEL_FILE OUT_FILE SEED N_STEPS DES_PER_STEP CARRY(0|1)
 //
@@ -66,22 +68,14 @@ int main(int argc, char** argv) {
         const uint64_t per = std::strtoull(argv[5], nullptr, 10);
         const bool carry = std::atoi(argv[6]) != 0;
 
+        const std::vector<std::string> problems = mfg::Simulation::SanityCheckModel(v);
+        if (!problems.empty()) throw std::runtime_error("model check: " + problems.front());
         mfg::Simulation sim(0);
         sim.LoadSimulation(v);
         sim.RebuildAccelStructure();
         sim.ResetSimulation(seed);
         sim.SetCarry(carry);
-        uint64_t remaining = steps * per, in_flight_max = 0;
-        for (;;) {
-            mfg::MCStepResult r = sim.SimulationMCStep(per, remaining, 1000000);
-            if (r == mfg::MCStepResult::DesorptionError) throw std::runtime_error("desorption error");
-            remaining -= per < remaining ? per : remaining;
-            uint64_t fl = sim.InFlight();
-            if (fl > in_flight_max) in_flight_max = fl;
-            if (r == mfg::MCStepResult::MaxReached) break;
-        }
-        sim.Drain(1000000);
-
+        // the host's global state (GlobalSimuState role)
         mfg_tally_layout L = sim.Layout();
         mfg_global_counters g{};
         std::vector<mfg_facet_counters> fc((size_t)L.n_slots * L.n_facets);
@@ -90,7 +84,18 @@ int main(int argc, char** argv) {
         tv.struct_size = sizeof(tv);
         tv.global = &g;
         tv.facets = fc.data();
-        sim.UpdateMCHits(tv);
+        uint64_t remaining = steps * per, in_flight_max = 0;
+        for (;;) {
+            mfg::MCStepResult r = sim.SimulationMCStep(per, remaining, 1000000);
+            if (r == mfg::MCStepResult::DesorptionError) throw std::runtime_error("desorption error");
+            remaining -= per < remaining ? per : remaining;
+            uint64_t fl = sim.InFlight();
+            if (fl > in_flight_max) in_flight_max = fl;
+            sim.UpdateHitsAndLog(tv);           // periodic report
+            if (r == mfg::MCStepResult::MaxReached) break;
+        }
+        sim.Drain(1000000);
+        sim.UpdateHitsAndLog(tv);
 
         std::FILE* out = std::fopen(argv[2], "w");
         if (!out) throw std::runtime_error("cannot write output");

@@ -60,9 +60,12 @@ def stored_facets(d: dict, slot: int = 0) -> dict:
 
 
 class StoredState:
-    """Stored results shaped like Tallies (for compare_states)."""
+    """Stored results shaped like Tallies, with the per-facet accessors of
+    molflow_sources_amd.compare.TallyState (for compare_states): profiles
+    of every stored moment, textures and histograms of the steady state."""
 
     def __init__(self, d: dict):
+        self._r = d["results"]
         from molflow_sources_amd.tallies import FACET_DT, GLOBAL_DT
         r = d["results"]
         g = r["global"]
@@ -77,6 +80,35 @@ class StoredState:
         for s in range(nS):
             for k, v in stored_facets(d, s).items():
                 self.facets[s][k] = v
+
+    def profile(self, f: int, m: int):
+        p = self._r["profiles"].get(str(f))
+        if p is None or m >= len(p):
+            return None
+        a = np.asarray(p[m], dtype=np.float64)
+        return a[:, 0], a[:, 1], a[:, 2]
+
+    def texture(self, f: int, m: int):
+        t = self._r["textures"].get(str(f))
+        if t is None or m != 0:
+            return None
+        return (np.asarray(t["count"], dtype=np.float64), np.asarray(t["sum_v_ort"], dtype=np.float64),
+                np.asarray(t["sum_1_per_v"], dtype=np.float64))
+
+    def direction(self, f: int, m: int):
+        return None
+
+    def facet_hist(self, f: int, m: int):
+        h = self._r["facet_hist"].get(str(f))
+        if h is None or m != 0:
+            return None
+        return tuple(np.asarray(h.get(k, []), dtype=np.float64) for k in ("Bounces", "Distance", "Time"))
+
+    def global_hist(self, m: int):
+        h = self._r["global_hist"]
+        if not h or m != 0:
+            return None
+        return tuple(np.asarray(h.get(k, []), dtype=np.float64) for k in ("Bounces", "Distance", "Time"))
 
 
 ALL_FIXTURES = sorted(n[:-8] for n in os.listdir(GOLDEN) if n.endswith(".json.gz"))

@@ -138,3 +138,85 @@ def test_microbench_library_exports_its_header():
     lib = ctypes.CDLL(lib_path)
     for n in names:
         assert hasattr(lib, n)
+
+
+def _bad_models():
+    """(name, model) pairs that the loader must refuse."""
+    out = []
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["super_idx"][5] = -2          # below the "every structure" value
+    out.append(("super_idx -2", m))
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["super_idx"][0] = 1           # the only structure is 0
+    out.append(("super_idx past the structures", m))
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["teleport_dest"][3] = -3      # -1 is the lowest meaning
+    out.append(("teleport_dest -3", m))
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["teleport_dest"][3] = m.n_facets + 1
+    out.append(("teleport_dest past the facets", m))
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["super_dest"][3] = 2          # link to a structure that does not exist
+    out.append(("super_dest past the structures", m))
+    return out
+
+
+@pytest.mark.parametrize("name,model", _bad_models(), ids=[n for n, _ in _bad_models()])
+def test_out_of_range_indices_are_refused_at_load(oracle_lib, name, model):
+    """The library's load-time preparation (through mfg_sanity_check, which
+    runs it without a device) and the oracle both refuse facet indices that
+    the kernel would otherwise use to index device tables."""
+    from molflow_sources_amd.simulation import sanity_check
+    problems = sanity_check(model)
+    assert problems, name
+    with pytest.raises(ValueError):
+        CpuModel(model)
+
+
+def test_sanity_check_accepts_good_models_and_reports_reference_checks(oracle_lib):
+    from molflow_sources_amd.simulation import sanity_check
+    assert sanity_check(geometry.tube_model(n_az=8, n_ax=2)) == []
+    assert sanity_check(geometry.time_dependent_tube()) == []
+    # MolflowSimulationModel::SanityCheck (MolflowSimGeom.cpp:337-404) items
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["count_abs"][1] = 1           # texture counter on an untextured facet
+    assert any("Untextured facet with texture counters" in p for p in sanity_check(m))
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.facets["outgassing"][0] = 0.0
+    m.facets["desorb_type"][0] = 0
+    probs = sanity_check(m)
+    assert any("No desorption facet found" in p for p in probs)
+    assert any("desorbing facet" in p for p in probs)   # the load itself fails too
+    m = geometry.tube_model(n_az=8, n_ax=2)
+    m.settings.enable_decay = 1
+    m.settings.half_life = 0.0
+    assert any("half life" in p for p in sanity_check(m))
+
+
+def test_oracle_adjoint_refuses_sticking_outside_the_open_interval(oracle_lib):
+    """d/ds = -n/(1-s) + 1/s is undefined at s = 0 and s = 1: the oracle's
+    adjoint refuses such group values instead of returning NaN or dropping
+    the -n/(1-s) term."""
+    import ctypes as C2
+    from oracle import mfcpu
+    m = geometry.beam_pipe_model(n_az=16, n_ax=20, port_rings=2, n_textures=0, n_anglemaps=0)
+    m.settings.low_flux_mode = 1
+    g, s, t = geometry.adjoint_setup(m, n_groups=4, n_targets=8)
+    cm = CpuModel(m)
+    L = mfcpu.lib()
+    j0, jac = np.zeros(8), np.zeros(32)
+    gi, ti = np.ascontiguousarray(g, np.int32), np.ascontiguousarray(t, np.int32)
+    for bad in (0.0, 1.0):
+        sv = np.array([0.01, bad, 0.02, 0.03])
+        rc = L.mfcpu_adjoint_run(cm.h, 42, 0, 10, gi.ctypes.data_as(C2.POINTER(C2.c_int32)), 4,
+                                 sv.ctypes.data_as(C2.POINTER(C2.c_double)),
+                                 ti.ctypes.data_as(C2.POINTER(C2.c_int32)), 8, 1,
+                                 j0.ctypes.data_as(C2.POINTER(C2.c_double)),
+                                 jac.ctypes.data_as(C2.POINTER(C2.c_double)), 1000)
+        assert rc != 0, bad
+    sv = np.array([0.01, 0.5, 0.02, 0.03])
+    rc = L.mfcpu_adjoint_run(cm.h, 42, 0, 10, gi.ctypes.data_as(C2.POINTER(C2.c_int32)), 4,
+                             sv.ctypes.data_as(C2.POINTER(C2.c_double)), ti.ctypes.data_as(C2.POINTER(C2.c_int32)), 8, 1,
+                             j0.ctypes.data_as(C2.POINTER(C2.c_double)), jac.ctypes.data_as(C2.POINTER(C2.c_double)),
+                             1000)
+    assert rc == 0 and np.all(np.isfinite(jac))

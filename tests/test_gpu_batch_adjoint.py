@@ -266,3 +266,33 @@ def test_hot_facets_from_autotune_keep_tube_tallies(sim):
     assert np.array_equal(a.facets["nb_desorbed"], b.facets["nb_desorbed"])
     for f in ("nb_abs_equiv", "sum_1_per_ort_velocity", "sum_v_ort", "sum_1_per_velocity"):
         assert np.allclose(a.facets[f], b.facets[f], rtol=1e-12, atol=0)
+
+
+def test_adjoint_setup_checks_group_sticking(sim):
+    """A group whose facets disagree on sticking is refused; a group whose
+    shared sticking is 0 or 1 (where -n/(1-s) or 1/s is undefined) blocks
+    runs until mfg_adjoint_set_sticking gives it a value inside (0,1)."""
+    from molflow_sources_amd.simulation import MolflowGpuError
+    m, grp, base, tg = adjoint_model()
+    g0 = np.nonzero(grp == 0)[0]
+    m2 = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+    m2.settings.low_flux_mode = 1
+    m2.facets["sticking"][g0[0]] = 0.02    # the rest of group 0 keeps 0.01
+    sim.load(m2)
+    sim.rebuild_accel()
+    with pytest.raises(MolflowGpuError, match="mixes facets"):
+        sim.adjoint_setup(grp, tg)
+    for s0 in (0.0, 1.0):
+        m3 = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+        m3.settings.low_flux_mode = 1
+        m3.facets["sticking"][grp == 1] = s0
+        sim.load(m3)
+        sim.rebuild_accel()
+        sim.reset(seed=3)
+        sim.adjoint_setup(grp, tg)
+        with pytest.raises(MolflowGpuError, match="inside"):
+            sim.run(100, max_events=100)
+        sim.adjoint_set_sticking(base)
+        sim.run(2000, max_events=200)
+        j0, jac = sim.adjoint_read()
+        assert np.all(np.isfinite(jac)) and np.all(np.isfinite(j0))

@@ -200,3 +200,77 @@ def test_carry_refused_with_adjoint(sim):
     sim.adjoint_setup(grp, tg)
     with pytest.raises(MolflowGpuError):
         sim.set_carry(True)
+
+
+@pytest.mark.parametrize("name", ["tube", "beam"])
+def test_take_after_each_carry_run_sums_to_one_run(sim, name):
+    """The periodic-report path of the boundary: carry on, k runs with a
+    take (add into the caller's state, then clear the device tallies)
+    after each, a drain and a last take. Parked particles survive every
+    take, so the summed state equals one uninterrupted run exactly in its
+    integer tallies (the UpdateHitsAndLog contract, Particle.cpp:1294-1312)."""
+    if name == "tube":
+        m, n, k = geometry.tube_model(), 300_000, 4
+    else:
+        m, n, k = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=4, n_anglemaps=2), 50_000, 3
+    sim.load(m)
+    sim.rebuild_accel()
+    ref = one_shot(sim, n * k, seed=17)
+    sim.reset(seed=17)
+    sim.set_carry(True)
+    acc = None
+    parked = []
+    for _ in range(k):
+        r = sim.run(n, max_events=MAXEV)
+        assert r.desorbed == n
+        parked.append(sim.in_flight())
+        acc = sim.take(acc)
+        # the device state is empty after a take, the parked particles are not
+        assert int(sim.read().glob[0]["nb_mc_hit"]) == 0
+        assert sim.in_flight() == parked[-1]
+    assert max(parked) > 0
+    sim.drain(max_events=MAXEV)
+    acc = sim.take(acc)
+    assert sim.in_flight() == 0
+    assert_same(snapshot(acc), ref)
+
+
+def test_time_limit_keeps_parked_particles_it_never_resumed(sim):
+    """Particles parked by a large grid, resumed by a much smaller one that
+    hits its time limit before it claims them all: the ones it never
+    claimed stay parked (none is lost), and in-flight plus ended equals
+    what was parked before."""
+    m = geometry.tube_model()
+    m.facets["sticking"][:2] = 0.0
+    m.facets["sticking"][2:] = 2e-4        # particles end, but only after thousands of bounces
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.reset(seed=9)
+    sim.set_carry(True)
+    try:
+        sim.set_time_limit(0.2)
+        r1 = sim.run(20_000_000, max_events=0)
+        assert r1.stopped == 2
+        before = sim.in_flight()
+        g0 = sim.read().glob[0]
+        ended0 = int(g0["nb_abs_equiv"]) + int(g0["nb_leak_total"])
+        assert before + ended0 == r1.desorbed
+        sim.set_launch(64, 1)                  # 64 lanes per CU: far fewer than parked
+        sim.set_time_limit(0.05)
+        r2 = sim.run(0, max_events=0)
+        assert r2.stopped == 2 and r2.desorbed == 0
+        after = sim.in_flight()
+        g1 = sim.read().glob[0]
+        ended1 = int(g1["nb_abs_equiv"]) + int(g1["nb_leak_total"])
+        assert after > 64 * 256, "the small grid should not have claimed every parked particle"
+        assert after + (ended1 - ended0) == before
+        # a full drain ends every one of them
+        sim.set_launch(256, 0)
+        sim.set_time_limit(60.0)
+        sim.drain(max_events=0)
+        g2 = sim.read().glob[0]
+        assert int(g2["nb_abs_equiv"]) + int(g2["nb_leak_total"]) == r1.desorbed
+    finally:
+        sim.set_launch(256, 0)
+        sim.set_time_limit(60.0)
+        sim.set_carry(False)

@@ -8,7 +8,7 @@ import pytest
 from fixture_models import StoredState, model_from_fixture
 from helpers import canonical_trace, polygon_zoo_model, random_rays, z_two_proportions
 from molflow_sources_amd import geometry
-from molflow_sources_amd.compare import absorption_chi2, compare_states
+from molflow_sources_amd.compare import TallyState, absorption_chi2, compare_states, results_wrong_detection
 from oracle.mfcpu import CpuModel
 
 pytestmark = pytest.mark.gpu
@@ -273,8 +273,29 @@ def test_gpu_compare_with_stored_results(gpu, name, n):
     r = np.array(ratios)
     se = r.std(ddof=1) / math.sqrt(batches)
     S = StoredState(d)
-    g_err, f_err, msgs = compare_states(t, S, glob=max(0.009, 4 * se / r.mean()), loc=0.07)
+    cnt = {}
+    g_err, f_err, fine, msgs = compare_states(TallyState(t, m), S, glob=max(0.009, 4 * se / r.mean()), loc=0.07,
+                                              counts=cnt)
     assert g_err == 0 and f_err == 0, msgs[:5]
+    assert fine <= 0.1 * cnt["fine_checks"] + 2, (fine, cnt)
+
+
+@pytest.mark.parametrize("name", ["B02-lr10_pipe_tex", "01-quick_pipe_profiles_textures_2sided", "09-histograms",
+                                  "05-three_structures_nonsquare_textures"])
+def test_gpu_results_wrong_protocol(gpu, name):
+    """The reference's negative control (ResultsWrong): 15 GPU runs of 300
+    desorptions must differ from the stored results in at least 66 % of the
+    comparisons at 0.006 / 0.05."""
+    m, d = model_from_fixture(name)
+    S = StoredState(d)
+
+    def short(i):
+        loaded(gpu, m, seed=500 + i)
+        gpu.run(300, max_events=MAXEV)
+        return TallyState(gpu.read(), m)
+
+    det, n, comparable = results_wrong_detection(short, S)
+    assert det / n >= 0.66, (det, n, comparable)
 
 
 def test_transmission_vs_reference_values(gpu):
