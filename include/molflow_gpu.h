@@ -367,6 +367,20 @@ int mfg_intersect_batch(mfg_context* ctx, uint64_t n, const double* origins, con
                         const int32_t* last_hit, int32_t structure,
                         int32_t* out_facet, double* out_t, double* out_u, double* out_v);
 
+/* mfg_intersect_batch without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as mfg_intersect_batch; the rays
+ * and results are staged through a fixed device array that the library
+ * holds on every GPU it runs on (about 5 MB, for MFG_STATIC_MAX_RAYS rays),
+ * so n above MFG_STATIC_MAX_RAYS is refused with MFG_ERR_LIMIT. Calls on
+ * one GPU share that array: calls that overlap in time on one GPU (from
+ * several contexts or threads) race and must be serialised by the caller. */
+#define MFG_STATIC_MAX_RAYS 65536
+int mfg_intersect_batch_static(mfg_context* ctx, uint64_t n, const double* origins, const double* dirs,
+                               const int32_t* last_hit, int32_t structure,
+                               int32_t* out_facet, double* out_t, double* out_u, double* out_v);
+
 /* ---- derived tables, exposed for checking ------------------------------ */
 /* Maxwell wall-collision speed CDF at 1 K (CDFGeneration.cpp:55-83):
  * 100 (velocity, cumulative probability) pairs. */

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libmolflow_gpu.so")
 
 MFG_ABI_VERSION = 1
 MFG_OK = 0
+MFG_ERR_LIMIT = 6
+STATIC_MAX_RAYS = 65536       # MFG_STATIC_MAX_RAYS
 
 DES_NONE, DES_UNIFORM, DES_COSINE, DES_COSINE_N, DES_ANGLEMAP = range(5)
 PROFILE_NONE, PROFILE_U, PROFILE_V, PROFILE_ANGULAR, PROFILE_VELOCITY, PROFILE_ORT_VELOCITY, PROFILE_TAN_VELOCITY = range(7)
@@ -148,6 +150,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mfg_sanity_check": (C.c_int, [C.POINTER(ModelView), C.c_char_p, C.c_size_t, C.POINTER(u32)]),
         "mfg_hits_size": (C.c_int, [vp, C.POINTER(u64)]),
         "mfg_intersect_batch": (C.c_int, [vp, u64, dp, dp, C.POINTER(i32), i32, C.POINTER(i32), dp, dp, dp]),
+        "mfg_intersect_batch_static": (C.c_int, [vp, u64, dp, dp, C.POINTER(i32), i32, C.POINTER(i32), dp, dp, dp]),
         "mfg_maxwell_cdf": (C.c_int, [C.c_double, dp, dp, C.c_size_t]),
         "mfg_parse_user_moment": (C.c_int, [C.c_char_p, C.c_double, dp, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mfg_integrated_desorption": (C.c_int, [dp, dp, C.c_size_t, C.c_int, C.c_int, C.c_double, dp, dp,
@@ -180,7 +183,7 @@ EXPORTED_SYMBOLS = (
     "mfg_create", "mfg_destroy", "mfg_last_error", "mfg_abi_version", "mfg_load_model", "mfg_build_accel",
     "mfg_reset", "mfg_run", "mfg_request_stop", "mfg_set_time_limit", "mfg_get_layout", "mfg_get_facet_offsets", "mfg_read_tallies",
     "mfg_take_tallies", "mfg_clear_tallies", "mfg_reinit_particle_log", "mfg_sanity_check",
-    "mfg_hits_size", "mfg_intersect_batch", "mfg_maxwell_cdf", "mfg_parse_user_moment",
+    "mfg_hits_size", "mfg_intersect_batch", "mfg_intersect_batch_static", "mfg_maxwell_cdf", "mfg_parse_user_moment",
     "mfg_integrated_desorption", "mfg_set_trace", "mfg_get_trace", "mfg_load_variants", "mfg_run_variants",
     "mfg_get_variant_layout", "mfg_read_variant_tallies", "mfg_set_launch", "mfg_enable_graph",
     "mfg_set_kernel_variant", "mfg_autotune", "mfg_set_carry", "mfg_in_flight",

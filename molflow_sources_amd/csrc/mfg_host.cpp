@@ -878,6 +878,10 @@ struct mfg_context {
     hipGraph_t graph = nullptr;
     std::vector<uint64_t> graph_key;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // staging array of mfg_intersect_batch_static: the device address of
+    // the library's static `scratch` array on this context's device
+    StaticRays* scratch = nullptr;
+    std::vector<int32_t> h_roots;        // host copy of the BVH roots
 };
 
 namespace {
@@ -1224,6 +1228,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
     c->leaves.upload(lf, s);
     c->leaf_uv.upload(luv, s);
     c->roots.upload(roots, s);
+    c->h_roots = roots;
     HIPCHK(hipStreamSynchronize(s));
     c->accel = true;
     reset_graph(c);
@@ -1537,6 +1542,30 @@ void load_all(mfg_context* c, const mfg_model_view* const* views, uint32_t n) {
 
 }  // namespace
 
+namespace {
+// Checks shared by both fixed-ray-batch entry points.
+void check_batch(mfg_context* c, uint64_t n, const double* o, const double* d, const int32_t* last, int32_t structure,
+                 int32_t* of, double* ot, double* ou, double* ov) {
+    if (!c->accel) throw Err(MFG_ERR_STATE, "acceleration structure not built");
+    if (!o || !d || !of || !ot || !ou || !ov) throw Err(MFG_ERR_INVALID, "null buffer");
+    if (structure < 0 || (uint32_t)structure >= c->dvars[0].n_struct) throw Err(MFG_ERR_INVALID, "structure out of range");
+    if (last)
+        for (uint64_t i = 0; i < n; ++i)
+            if (last[i] >= (int32_t)c->dvars[0].n_facets) throw Err(MFG_ERR_INVALID, "last_hit out of range");
+}
+
+// First hits of n rays whose inputs and outputs sit in device buffers.
+void launch_batch(mfg_context* c, uint64_t n, const double* dO, const double* dD, const int32_t* dL, int32_t structure,
+                  int32_t* dF, double* dT, double* dU, double* dV) {
+    IsectParams p{};
+    p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p;
+    p.root = c->h_roots.at(c->dvars[0].root_base + (uint32_t)structure);
+    p.stack_size = c->stack_size; p.n = n; p.origins = dO; p.dirs = dD; p.last_hit = dL;
+    p.out_facet = dF; p.out_t = dT; p.out_u = dU; p.out_v = dV; p.facet_base = c->dvars[0].facet_base;
+    if (mfg_launch_intersect(p, c->stream)) throw Err(MFG_ERR_DEVICE, "intersect launch failed");
+}
+}  // namespace
+
 extern "C" {
 
 int mfg_abi_version(void) { return MFG_ABI_VERSION; }
@@ -1562,6 +1591,7 @@ int mfg_create(mfg_context** out, int device) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
         return MFG_ERR_DEVICE;
     c->clock_khz = (uint64_t)khz;
+    if (mfg_scratch_address(device, &c->scratch) != 0 || !c->scratch) return MFG_ERR_DEVICE;
     *out = c.release();
     return MFG_OK;
 }
@@ -1745,15 +1775,8 @@ int mfg_hits_size(const mfg_context* c, uint64_t* bytes) {
 int mfg_intersect_batch(mfg_context* c, uint64_t n, const double* o, const double* d, const int32_t* last,
                         int32_t structure, int32_t* of, double* ot, double* ou, double* ov) {
     return guarded(c, [&] {
-        if (!c->accel) throw Err(MFG_ERR_STATE, "acceleration structure not built");
-        if (!o || !d || !of || !ot || !ou || !ov) throw Err(MFG_ERR_INVALID, "null buffer");
-        if (structure < 0 || (uint32_t)structure >= c->dvars[0].n_struct) throw Err(MFG_ERR_INVALID, "structure out of range");
-        if (last)
-            for (uint64_t i = 0; i < n; ++i)
-                if (last[i] >= (int32_t)c->dvars[0].n_facets) throw Err(MFG_ERR_INVALID, "last_hit out of range");
+        check_batch(c, n, o, d, last, structure, of, ot, ou, ov);
         if (n == 0) return;
-        std::vector<int32_t> roots(c->roots.n);
-        HIPCHK(hipMemcpy(roots.data(), c->roots.p, roots.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
         DevBuf<double> dO, dD, dT, dU, dV;
         DevBuf<int32_t> dL, dF;
         dO.alloc(3 * n); dD.alloc(3 * n); dT.alloc(n); dU.alloc(n); dV.alloc(n); dF.alloc(n);
@@ -1761,16 +1784,36 @@ int mfg_intersect_batch(mfg_context* c, uint64_t n, const double* o, const doubl
         HIPCHK(hipMemcpyAsync(dO.p, o, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dD.p, d, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
         if (last) { dL.alloc(n); HIPCHK(hipMemcpyAsync(dL.p, last, n * sizeof(int32_t), hipMemcpyHostToDevice, s)); }
-        IsectParams p{};
-        p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p;
-        p.root = roots[c->dvars[0].root_base + (uint32_t)structure];
-        p.stack_size = c->stack_size; p.n = n; p.origins = dO.p; p.dirs = dD.p; p.last_hit = last ? dL.p : nullptr;
-        p.out_facet = dF.p; p.out_t = dT.p; p.out_u = dU.p; p.out_v = dV.p; p.facet_base = c->dvars[0].facet_base;
-        if (mfg_launch_intersect(p, s)) throw Err(MFG_ERR_DEVICE, "intersect launch failed");
+        launch_batch(c, n, dO.p, dD.p, last ? dL.p : nullptr, structure, dF.p, dT.p, dU.p, dV.p);
         HIPCHK(hipMemcpyAsync(of, dF.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(ot, dT.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(ou, dU.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(ov, dV.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int mfg_intersect_batch_static(mfg_context* c, uint64_t n, const double* o, const double* d, const int32_t* last,
+                               int32_t structure, int32_t* of, double* ot, double* ou, double* ov) {
+    return guarded(c, [&] {
+        if (n > MFG_STATIC_MAX_RAYS)
+            throw Err(MFG_ERR_LIMIT, "mfg_intersect_batch_static takes at most " + std::to_string(MFG_STATIC_MAX_RAYS) +
+                                         " rays per call");
+        check_batch(c, n, o, d, last, structure, of, ot, ou, ov);
+        if (n == 0) return;
+        // every buffer below is a field of the static staging array; nothing
+        // is allocated on the device (the outputs are written in full, so
+        // nothing needs clearing between calls)
+        StaticRays* S = c->scratch;
+        hipStream_t s = c->stream;
+        HIPCHK(hipMemcpyAsync(S->origins, o, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(S->dirs, d, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (last) HIPCHK(hipMemcpyAsync(S->last_hit, last, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        launch_batch(c, n, S->origins, S->dirs, last ? S->last_hit : nullptr, structure, S->facet, S->t, S->u, S->v);
+        HIPCHK(hipMemcpyAsync(of, S->facet, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(ot, S->t, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(ou, S->u, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(ov, S->v, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     });
 }

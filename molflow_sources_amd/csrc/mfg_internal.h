@@ -323,6 +323,20 @@ struct IsectParams {
     uint32_t facet_base;
 };
 
+// Staging layout of the allocation-free intersect entry: the library's
+// static device array `scratch` (mfg_trace.hip) holds one of these
+#define MFG_MAX_DEVICES 64
+struct StaticRays {
+    double origins[3 * MFG_STATIC_MAX_RAYS];
+    double dirs[3 * MFG_STATIC_MAX_RAYS];
+    double t[MFG_STATIC_MAX_RAYS], u[MFG_STATIC_MAX_RAYS], v[MFG_STATIC_MAX_RAYS];
+    int32_t last_hit[MFG_STATIC_MAX_RAYS];
+    int32_t facet[MFG_STATIC_MAX_RAYS];
+};
+// device address of `scratch` on `device`, which must be the current
+// device; looked up once per device (hipGetSymbolAddress), 0 on success
+int mfg_scratch_address(int device, StaticRays** out);
+
 // kernel launchers (mfg_trace.hip)
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream);
 int mfg_launch_intersect(const IsectParams& p, void* stream);

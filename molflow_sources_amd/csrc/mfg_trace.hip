@@ -16,6 +16,8 @@
 // CPU restatement, so seeded runs are comparable event by event.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mfg_internal.h"
 
 namespace {
@@ -1901,6 +1903,33 @@ int mfg_launch_zero(void* ptr, uint64_t bytes, void* stream) {
     if (grid > 4096) grid = 4096;
     hipLaunchKernelGGL(k_zero, dim3((uint32_t)grid), dim3(256), 0, (hipStream_t)stream, (uint64_t*)ptr, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Device staging of the allocation-free fixed-ray-batch entry
+// (mfg_intersect_batch_static): origins, directions, skip ids and the four
+// outputs of up to MFG_STATIC_MAX_RAYS rays, at the fixed offsets of
+// StaticRays. Every GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address that
+// mfg_scratch_address returns for the current device.
+static __device__ __align__(256) unsigned char scratch[sizeof(StaticRays)];
+
+namespace {
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MFG_MAX_DEVICES] = {};
+}  // namespace
+
+int mfg_scratch_address(int device, StaticRays** out) {
+    if (!out || device < 0 || device >= MFG_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return -1;   // the device must be current
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<StaticRays*>(g_scratch_ptr[device]);
+    return 0;
 }
 
 int mfg_launch_backward(const double* jac, const double* up, double* grad, uint32_t nt, uint32_t ng, double scale,

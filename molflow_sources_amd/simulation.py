@@ -208,6 +208,20 @@ class GpuSimulation:
                     "mfg_intersect_batch")
         return f, t, u, v
 
+    def intersect_static(self, origins, dirs, last_hit=None, structure: int = 0):
+        """intersect() through the entry point that allocates no device
+        memory (at most STATIC_MAX_RAYS rays per call)."""
+        o = np.ascontiguousarray(origins, dtype=np.float64)
+        d = np.ascontiguousarray(dirs, dtype=np.float64)
+        n = len(o)
+        lh = None if last_hit is None else np.ascontiguousarray(last_hit, dtype=np.int32)
+        f = np.zeros(n, dtype=np.int32)
+        t, u, v = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._check(self._lib.mfg_intersect_batch_static(self.h, n, _dp(o), _dp(d), None if lh is None else _ip(lh),
+                                                         structure, _ip(f), _dp(t), _dp(u), _dp(v)),
+                    "mfg_intersect_batch_static")
+        return f, t, u, v
+
     def set_trace(self, n: int, k: int) -> None:
         self._trace_shape = (n, k)
         self._check(self._lib.mfg_set_trace(self.h, n, k), "mfg_set_trace")
