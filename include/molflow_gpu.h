@@ -355,6 +355,14 @@ int mfg_reinit_particle_log(mfg_context* ctx);
  * the check ran; n_problems == 0 means the model loads. */
 int mfg_sanity_check(const mfg_model_view* model, char* out, size_t cap, uint32_t* n_problems);
 
+/* The model totals the physical quantities are scaled with
+ * (CalcTotalOutgassing, MolflowSimGeom.cpp:290-335; read by
+ * GetMoleculesPerTP, CSVExporter.cpp:59-79): out[0] final outgassing rate
+ * (molecules/s), out[1] the same in Pa m3/s, out[2] total desorbed
+ * molecules up to the latest moment, out[3] the latest moment (s). Needs no
+ * device. Returns the error mfg_load_model would give for the view. */
+int mfg_model_totals(const mfg_model_view* model, double out[4]);
+
 /* Bytes of one full tally state (GetHitsSize, MolflowSimulation.cpp:156-160). */
 int mfg_hits_size(const mfg_context* ctx, uint64_t* bytes);
 

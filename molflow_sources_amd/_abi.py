@@ -148,6 +148,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mfg_clear_tallies": (C.c_int, [vp]),
         "mfg_reinit_particle_log": (C.c_int, [vp]),
         "mfg_sanity_check": (C.c_int, [C.POINTER(ModelView), C.c_char_p, C.c_size_t, C.POINTER(u32)]),
+        "mfg_model_totals": (C.c_int, [C.POINTER(ModelView), dp]),
         "mfg_hits_size": (C.c_int, [vp, C.POINTER(u64)]),
         "mfg_intersect_batch": (C.c_int, [vp, u64, dp, dp, C.POINTER(i32), i32, C.POINTER(i32), dp, dp, dp]),
         "mfg_intersect_batch_static": (C.c_int, [vp, u64, dp, dp, C.POINTER(i32), i32, C.POINTER(i32), dp, dp, dp]),
@@ -182,7 +183,7 @@ def load_library(path: str | None = None) -> C.CDLL:
 EXPORTED_SYMBOLS = (
     "mfg_create", "mfg_destroy", "mfg_last_error", "mfg_abi_version", "mfg_load_model", "mfg_build_accel",
     "mfg_reset", "mfg_run", "mfg_request_stop", "mfg_set_time_limit", "mfg_get_layout", "mfg_get_facet_offsets", "mfg_read_tallies",
-    "mfg_take_tallies", "mfg_clear_tallies", "mfg_reinit_particle_log", "mfg_sanity_check",
+    "mfg_take_tallies", "mfg_clear_tallies", "mfg_reinit_particle_log", "mfg_sanity_check", "mfg_model_totals",
     "mfg_hits_size", "mfg_intersect_batch", "mfg_intersect_batch_static", "mfg_maxwell_cdf", "mfg_parse_user_moment",
     "mfg_integrated_desorption", "mfg_set_trace", "mfg_get_trace", "mfg_load_variants", "mfg_run_variants",
     "mfg_get_variant_layout", "mfg_read_variant_tallies", "mfg_set_launch", "mfg_enable_graph",

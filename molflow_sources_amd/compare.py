@@ -56,9 +56,10 @@ class TallyState:
         self.t = tallies
         self.glob = tallies.glob
         self.facets = tallies.facets
+        from .model import texture_cells
         F = model.facets
-        self.tex_w = np.ceil(F["tex_width_precise"]).astype(np.int64)
-        self.tex_h = np.ceil(F["tex_height_precise"]).astype(np.int64)
+        self.tex_w = np.array([texture_cells(x) for x in F["tex_width_precise"]], dtype=np.int64)
+        self.tex_h = np.array([texture_cells(x) for x in F["tex_height_precise"]], dtype=np.int64)
         s = model.settings
 
         def hist_sizes(rb, bb, bm, rd, db, dm, rt, tb, tm):

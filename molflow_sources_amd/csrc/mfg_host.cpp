@@ -29,6 +29,17 @@ constexpr double kMbarlsToPam3s = 0.1;
 constexpr int kMaxwellPoints = 100;
 constexpr uint32_t kLeakCap = 2048;
 
+// Texture cells along one side for a precise size: the ceiling, except
+// that a size above a whole number by rounding noise only (relative 1e-9)
+// counts as that number. The rule of MolFlow's own (absent) facet code as
+// its stored results show it: test case 02c stores texDimY
+// 5.000000000000002 with 5 rows of cells.
+int32_t texture_cells(double precise) {
+    int32_t n = (int32_t)std::ceil(precise);
+    if (n > 1 && precise - (n - 1) <= 1e-9 * precise) --n;
+    return n;
+}
+
 struct Err : std::runtime_error {
     int code;
     Err(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -402,7 +413,7 @@ struct HostModel {
     std::vector<int32_t> src_facet;   // local
     std::vector<double> og_cdf;
     std::vector<double> am_src;       // DES_ANGLEMAP blocks (variant-local offsets in DPhys)
-    double latest = 0, total_des = 0, final_rate = 0, c_factor = 1.0;
+    double latest = 0, total_des = 0, final_rate = 0, final_rate_pam3s = 0, c_factor = 1.0;
     uint32_t n_slots = 1;
     uint64_t n_tex = 0, n_dir = 0, n_prof = 0, n_am = 0, n_fh = 0;
     uint32_t gh_b = 0, gh_d = 0, gh_t = 0;
@@ -664,8 +675,8 @@ void prepare(HostModel& M, const mfg_model_view* v) {
         // textures
         if (d.is_textured && d.tex_width_precise > 0 && d.tex_height_precise > 0) {
             fl |= FF_TEXTURE;
-            ph.tex_w = (int32_t)std::ceil(d.tex_width_precise);
-            ph.tex_h = (int32_t)std::ceil(d.tex_height_precise);
+            ph.tex_w = texture_cells(d.tex_width_precise);
+            ph.tex_h = texture_cells(d.tex_height_precise);
             ph.tex_wp = d.tex_width_precise; ph.tex_hp = d.tex_height_precise;
             ph.tex_off = (int64_t)M.n_tex;
             if (d.count_dir) { ph.dir_off = (int64_t)M.n_dir; M.n_dir += (uint64_t)ph.tex_w * ph.tex_h; }
@@ -736,16 +747,20 @@ void prepare(HostModel& M, const mfg_model_view* v) {
             double sum = M.og_cdf[ph.og_off + (uint64_t)ph.og_w * ph.og_h - 1];
             M.total_des += M.latest * sum / kT;
             M.final_rate += sum / kT;
+            M.final_rate_pam3s += sum;
             if (!(d.total_outgassing > 0)) continue;
             w = M.latest * d.total_outgassing / kT;
         } else if (ph.outgassing_param < 0) {
             w = M.latest * d.outgassing / kT;
             M.total_des += w;
             M.final_rate += d.outgassing / kT;
+            M.final_rate_pam3s += d.outgassing;
         } else {
             w = M.IDs[ph.id_index].back().y / kT;
             M.total_des += w;
-            M.final_rate += M.params[ph.outgassing_param].pts.back().y * kMbarlsToPam3s / kT;
+            const double last = M.params[ph.outgassing_param].pts.back().y * kMbarlsToPam3s;
+            M.final_rate += last / kT;
+            M.final_rate_pam3s += last;
         }
         if (w <= 0) continue;
         if (d.super_idx == -1) throw Err(MFG_ERR_INVALID, "facet " + std::to_string(f) + " is in all structures and can't desorb");
@@ -1754,6 +1769,23 @@ int mfg_sanity_check(const mfg_model_view* v, char* out, size_t cap, uint32_t* n
         out[n] = '\0';
     }
     *n_problems = (uint32_t)log.size();
+    return MFG_OK;
+}
+
+int mfg_model_totals(const mfg_model_view* v, double* out) {
+    if (!out) return MFG_ERR_INVALID;
+    try {
+        HostModel M;
+        prepare(M, v);
+        out[0] = M.final_rate;
+        out[1] = M.final_rate_pam3s;
+        out[2] = M.total_des;
+        out[3] = M.latest;
+    } catch (const Err& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return MFG_ERR_INVALID;
+    }
     return MFG_OK;
 }
 

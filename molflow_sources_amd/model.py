@@ -34,12 +34,27 @@ def default_facets(n: int) -> np.ndarray:
     return f
 
 
+def texture_cells(precise: float) -> int:
+    """Texture cells along one side for a precise size (texWidth from
+    texWidth_precise): the ceiling, except that a size exceeding a whole
+    number by rounding noise only (relative 1e-9) counts as that number.
+    MolFlow's own rule lives in its absent shared library; its stored
+    results (fixture 02c: texDimY 5.000000000000002, 5 rows) show this
+    behaviour."""
+    import math
+    n = math.ceil(precise)
+    if n > 1 and precise - (n - 1) <= 1e-9 * precise:
+        n -= 1
+    return int(n)
+
+
 def default_settings() -> Settings:
     s = Settings()
     s.gas_mass = 28.0
     s.half_life = 1.0
     s.time_window = 1e-10
     s.low_flux_cutoff = 1e-7
+    s.scatter_rho = 1.0      # the scattering laws depend on b / rho only (b = rho sqrt(u))
     s.use_maxwell = 1
     s.calc_constant_flow = 1
     s.log_facet_id = -1

@@ -94,6 +94,16 @@ void philox(const uint32_t c[4], const uint32_t k[2], uint32_t o[4]) {
 }
 
 inline double u32_to_unit(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+// Texture cells along one side for a precise size: the ceiling, except
+// that a size above a whole number by rounding noise only (relative 1e-9)
+// counts as that number. The rule of MolFlow's own (absent) facet code,
+// as its stored results show it: test case 02c stores texDimY
+// 5.000000000000002 with 5 rows of cells.
+inline int texture_cells(double precise) {
+    int n = (int)std::ceil(precise);
+    if (n > 1 && precise - (n - 1) <= 1e-9 * precise) --n;
+    return n;
+}
 
 struct Rng {
     uint32_t key[2];
@@ -747,8 +757,8 @@ void load_model(Model& M, const mfg_model_view* v) {
             if (pid >= (int)M.params.size()) throw std::runtime_error("facet refers to a missing parameter");
         // textures (Initializer.cpp:433-456)
         if (F.d.is_textured && F.d.tex_width_precise > 0 && F.d.tex_height_precise > 0) {
-            F.tex_w = (int)std::ceil(F.d.tex_width_precise);
-            F.tex_h = (int)std::ceil(F.d.tex_height_precise);
+            F.tex_w = texture_cells(F.d.tex_width_precise);
+            F.tex_h = texture_cells(F.d.tex_height_precise);
             double rw = lu / F.d.tex_width_precise, rh = lv / F.d.tex_height_precise;
             double full = rw * rh, two = F.d.is_2sided ? 2.0 : 1.0;
             F.tex_inc.resize((size_t)F.tex_w * F.tex_h);

@@ -141,6 +141,19 @@ def force_box(torque_ref=(1.0, 2.0, 3.0), absorber_sticking: float = 0.3) -> Mod
     return b.model(s, name="force_box")
 
 
+def equilibrium_box(pump_half: float = 0.5) -> Model:
+    """Closed isothermal cube with diffuse walls, a small two-sided pump
+    (s = 1) floating 2 cm above the floor and a tiny transparent source in
+    the middle: the walls see gas close to equilibrium, so their pressure,
+    density, impingement rate and mean speed obey P = n k T and
+    Z = n <v> / 4."""
+    b = cube(10.0)
+    b.add("pump", square(5.0, 5.0, 2.0, pump_half), [0, 0, 1], sticking=1.0, is_2sided=1)
+    b.add("source", square(5.0, 5.0, 5.0, 0.005), [0, 0, -1], desorb_type=A.DES_COSINE, outgassing=1.0,
+          opacity=0.0)
+    return b.model(default_settings(), name="equilibrium_box")
+
+
 def scatter_beam(mfp: float = 2.0, mass_ratio: float = 1.0, rho: float = 1.0, cutoff: float = 0.0) -> Model:
     """The thin box with an absorbing floor, a collimated source (cos^N,
     N = 1e5) 1 cm above it and background-gas scattering of mean free path
