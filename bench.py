@@ -12,16 +12,29 @@ drain launch that traces every remaining particle to its end, so every hit
 of the K*des-per-step desorptions is inside the timed region; the default
 100 steps of 1e6 is the full 1e8-desorption job.
 
---workload tube runs the plain tube instead (R 1 cm, L 10 cm, 64 x 50 wall
-quads, cosine inlet, outlet of sticking 1): 10 steps of 1e6, the whole
-1e7-desorption job, with its transmission probability and its own CPU
-baseline.
-
 Prints ONE JSON line: hits/s (reference definition: bounces + absorptions +
 volume scatters per wall second) over K timed steps, with the roofline
 figure of the trace kernel (algorithmic bytes per hit from the reference-
 equivalent BVH, HIP-event kernel time) and the CPU oracle timed on this
-host's cores on a bounded sample of the same workload.
+host's cores on a bounded sample of the same workload. The same line
+carries the other legs of the headline metric (skipped with --no-extra):
+  tube          the 1e7-desorption tube job (R 1 cm, L 10 cm, 64 x 50 wall
+                quads) and its own CPU baseline;
+  batched       64 uneven beam-pipe variants (a quarter of the 256 of the
+                batched workload, 2e3-2e4 desorptions each: a tenth) in
+                one launch, captured in a hipGraph, against one launch per
+                variant;
+  adjoint       cost of a run that also accumulates the 32 x 64 sticking
+                Jacobian against the same forward run without it;
+  matched_sigma time the GPU and the CPU oracle need to bring the standard
+                error of one tally (the fraction of particles the 8 pumps
+                absorb) to the same value.
+Counter-derived figures (HBM traffic, atomics, VALU issue) come from the
+rocprofv3 summaries under profiles/, and only when the summary records the
+hash of the kernel sources this build was made from.
+
+--workload tube / batched / adjoint run one leg alone (batched at 256
+variants of 2e4-2e5 desorptions; adjoint with its finite-difference check).
 """
 from __future__ import annotations
 
@@ -109,21 +122,44 @@ def cpu_baseline(model, run_s=5.0, repeats=3, name="beam-pipe"):
         total_s += dt
     rates.sort()
     return {"value": rates[len(rates) // 2], "unit": "hits/s", "cores": threads, "kind": "port",
+            "desorptions_per_s": n_run * repeats / total_s,
             "sample": f"median of {repeats} runs of {n_run} desorptions of the same {name} model "
                       f"({total_hits} hits in {total_s:.1f} s in all), oracle/mfcpu.cpp",
             "runs": rates}
 
 
-def traffic_from_profile():
-    """HBM bytes per trace launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration)."""
-    p = os.path.join(ROOT, "profiles", "trace_beam_pipe_pmc.txt")
+KERNEL_SOURCES = ("molflow_sources_amd/csrc/mfg_trace.hip", "molflow_sources_amd/csrc/mfg_internal.h",
+                  "include/molflow_gpu.h")
+
+
+def kernel_source_hash():
+    """sha256 of the kernel sources (the trace kernel and the layouts it
+    reads): a profile summary applies to this build only if it records it."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def profile_value(name, key):
+    """`key: value` from profiles/<name> when the summary's
+    kernel_source_sha256 line names these kernel sources, else None."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
+    vals = {}
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        if ":" in line and not line.startswith((" ", "#")):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("kernel_source_sha256") != kernel_source_hash() or key not in vals:
+        return None
+    try:
+        return float(vals[key])
+    except ValueError:
+        return None
 
 
 def atomic_peak(device=0, n_addr=1 << 20, per_thread=64):
@@ -146,29 +182,19 @@ def atomic_peak(device=0, n_addr=1 << 20, per_thread=64):
             "ops": "one FP64 (or uint64) atomic add without return per lane, 8 blocks of 256 per CU"}
 
 
-def atomics_from_profile():
-    """Global atomics of one steady-state trace launch, from the committed
-    rocprofv3 TCC_ATOMIC summary (scaled by the count it reads for the
-    microbenchmark's known number of atomics)."""
-    p = os.path.join(ROOT, "profiles", "trace_beam_pipe_atomics.txt")
-    if not os.path.exists(p):
-        return None
-    for line in open(p):
-        if line.startswith("atomics_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed launches (default: the whole job, 100 for plain, 10 for tube)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--des-per-step", type=int, default=1_000_000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain workload only: skip the tube, batched, adjoint and matched-sigma legs")
     ap.add_argument("--no-carry", action="store_true", help="trace every launch's particles to their end")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
+    ap.add_argument("--waves", type=int, default=0, help="occupancy bound of the trace kernel (with --no-tune)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -182,20 +208,22 @@ def main():
                     help="desorptions per variant, log-uniform in [lo,hi]")
     ap.add_argument("--adjoint-des", type=int, default=200_000)
     ap.add_argument("--fd-des", type=int, default=40_000)
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: runs across several GPUs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
-
     if args.workload == "batched":
-        return batched_bench(args)
+        lo, hi = (float(x) for x in args.variant_des.split(","))
+        print(json.dumps(batched_leg(args.variants, lo, hi)), flush=True)
+        return
     if args.workload == "adjoint":
-        return adjoint_bench(args)
-
-    from molflow_sources_amd import geometry
-    from molflow_sources_amd.simulation import GpuSimulation
-
+        print(json.dumps(adjoint_leg(args.adjoint_des, args.fd_des)), flush=True)
+        return
     if args.copy_calibration_mib:
         import torch
         n = args.copy_calibration_mib * (1 << 20) // 8
@@ -205,10 +233,27 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-
     tube = args.workload == "tube"
-    if args.steps is None:
-        args.steps = 10 if tube else 100
+    out, info = plain_leg(args, tube)
+    if not tube and not args.no_extra:
+        progress("tube leg")
+        out["tube"] = tube_leg(args)
+        progress("batched leg")
+        out["batched"] = batched_leg(64, 2e3, 2e4, graph=True)
+        progress("adjoint leg")
+        out["adjoint"] = adjoint_leg(100_000, 0)
+        if out["cpu_baseline"] and "pump_fraction_batches" in info:
+            out["matched_sigma"] = matched_sigma(info, out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+def plain_leg(args, tube):
+    """The timed job on the beam pipe (or the tube): returns the JSON line's
+    fields and what matched_sigma needs."""
+    from molflow_sources_amd import geometry
+    from molflow_sources_amd.simulation import GpuSimulation
+
+    steps = args.steps if args.steps is not None else (10 if tube else 100)
     if tube:
         # R 1 cm, L 10 cm, 64 x 50 wall quads, inlet disk desorbing (cosine),
         # outlet disk of sticking 1, walls of sticking 0: 1e7 desorptions
@@ -230,6 +275,8 @@ def main():
         tr = sim.autotune(args.des_per_step)
         tune = {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu,
                 "bvh_width": tr.bvh_width, "leaf_size": tr.leaf_size, "hot_facets": tr.n_hot, "probe_hits_per_s": tr.hits_per_s}
+    elif args.waves:
+        sim.set_kernel_variant(args.waves)
     max_events = 50_000_000
     for _ in range(args.warmup):
         sim.run(args.des_per_step, max_events=max_events)
@@ -239,7 +286,7 @@ def main():
     hits = 0
     kernel_ms = 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         r = sim.run(args.des_per_step, max_events=max_events)   # synchronous: returns after the device is done
         hits += r.hits
         kernel_ms += r.kernel_ms
@@ -250,26 +297,43 @@ def main():
     wall = time.perf_counter() - t0
     drain_s = time.perf_counter() - td
     t = sim.read()
-    assert t.glob[0]["nb_desorbed"] == args.steps * args.des_per_step and sim.in_flight() == 0
+    assert t.glob[0]["nb_desorbed"] == steps * args.des_per_step and sim.in_flight() == 0
     assert t.glob[0]["nb_mc_hit"] == hits
     value = hits / wall
+    info = {"des_per_s": steps * args.des_per_step / wall}
+    if not tube and not args.no_extra:
+        # batch means of the pump-absorbed fraction over 10 untimed batches
+        # (each traced to its end), for the matched-sigma leg
+        pumps = list(model.pump_ids)
+        fr = []
+        sim.set_carry(False)
+        for b in range(10):
+            sim.clear_tallies()
+            sim.reset(seed=1000 + b)
+            sim.run(args.des_per_step, max_events=max_events)
+            tb = sim.read()
+            fr.append(float(tb.facets[0]["nb_abs_equiv"][pumps].sum()) / args.des_per_step)
+        info.update(pump_fraction_batches=fr, batch_des=args.des_per_step)
+    sim.close()
     fh = t.facets["nb_mc_hit"][0].astype(float)
     tex_ids = list(getattr(model, "tex_ids", []))
     tex_frac = float(fh[tex_ids].sum() / max(fh.sum(), 1.0)) if tex_ids else 0.0
     b_hit, nodes, prims = algorithmic_bytes_per_hit(model, tex_frac, **ray_geom)
-    launches = args.steps + 1
+    launches = steps + 1
     launch_s = kernel_ms / 1e3 / launches
     hits_per_launch = hits / launches
     achieved = b_hit * hits_per_launch / launch_s / 1e9
+    name = "tube" if tube else "beam_pipe"
+    traffic = profile_value(f"trace_{name}_pmc.txt", "traffic_bytes_per_launch")
     out = {
         "metric": "MC hits/s on 1 MI355X (plain tracer, config-1 tube)" if tube else
                   "MC hits/s on 1 MI355X (plain tracer, config-2 beam pipe)",
         "value": value,
         "unit": "hits/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -278,17 +342,26 @@ def main():
         "config": {"workload": "tube_R1cm_L10cm_64x50_wall_quads" if tube else
                                "beam_pipe_100k_triangles_8_ports_16_textures_4_anglemaps",
                    "facets": model.n_facets, "desorptions_per_step": args.des_per_step,
-                   "hits_per_desorption": hits / (args.steps * args.des_per_step),
+                   "hits_per_desorption": hits / (steps * args.des_per_step),
                    "seed": 42, "parallelism": "single GPU", "carry": not args.no_carry,
                    "drain_ms": drain_s * 1e3, "launch": tune, "job": "steps x desorptions_per_step, every particle traced to its end"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None if tube else traffic_from_profile(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_hit": b_hit, "bvh_nodes_per_ray": nodes, "facet_tests_per_ray": prims,
-                     "kernel_ms_per_launch": launch_s * 1e3},
+                     "kernel_ms_per_launch": launch_s * 1e3,
+                     "traffic_source": f"profiles/trace_{name}_pmc.txt" if traffic else
+                                       "no summary of this kernel build under profiles/"},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(model, name="tube" if tube else "beam-pipe"),
     }
+    # the roof the counters show: VALU issue and memory wait of the trace kernel
+    valu = profile_value(f"trace_{name}_sq.txt", "valu_issue_per_cu")
+    wait = profile_value(f"trace_{name}_sq.txt", "wait_any_fraction")
+    if valu is not None:
+        out["issue"] = {"bound": "valu_issue", "achieved": valu, "peak": 1.0,
+                        "unit": "wave64 VALU instructions per CU cycle", "frac": valu,
+                        "wait_any_fraction": wait, "source": f"profiles/trace_{name}_sq.txt"}
     peak = atomic_peak()
-    atom = None if tube else atomics_from_profile()
+    atom = profile_value(f"trace_{name}_atomics.txt", "atomics_per_launch")
     if peak:
         out["atomics"] = {"bound": "l2_atomics", "achieved": atom / launch_s if atom else None,
                           "peak": peak["f64_per_s"], "unit": "atomic ops/s",
@@ -300,7 +373,44 @@ def main():
         out["config"]["transmission"] = float(t.facets["nb_abs_equiv"][0][1]) / float(t.glob[0]["nb_desorbed"])
     if out["cpu_baseline"]:
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
-    print(json.dumps(out), flush=True)
+    return out, info
+
+
+def tube_leg(args):
+    """The tube job (10 x 1e6 desorptions) and its CPU baseline, as extra
+    keys of the default line."""
+    a = parse_args(["--workload", "tube"] + (["--no-cpu"] if args.no_cpu else []))
+    out, _ = plain_leg(a, True)
+    r = {"metric": "MC hits/s, config-1 tube (R 1 cm, L 10 cm, 3,200 wall quads), 1e7 desorptions",
+         "value": out["value"], "unit": "hits/s", "kernel_ms_per_launch": out["roofline"]["kernel_ms_per_launch"],
+         "transmission": out["config"]["transmission"], "launch": out["config"]["launch"],
+         "roofline": out["roofline"], "cpu_baseline": out["cpu_baseline"]}
+    if "issue" in out:
+        r["issue"] = out["issue"]
+    if out["cpu_baseline"]:
+        r["speedup_vs_cpu"] = out["speedup_vs_cpu"]
+    return r
+
+
+def matched_sigma(info, cpu):
+    """Time for the GPU and for the CPU oracle to bring the standard error
+    of the pump-absorbed fraction p to 1e-5. Each particle ends on a pump
+    or not, so the per-particle variance is p (1 - p); the GPU's batch
+    means give it independently (both are reported)."""
+    import numpy as np
+    fr = np.array(info["pump_fraction_batches"])
+    n = info["batch_des"]
+    p = float(fr.mean())
+    v_bin = p * (1 - p)
+    v_bm = float(fr.var(ddof=1) * n)
+    target = 1e-5
+    gpu_s = v_bin / target ** 2 / info["des_per_s"]
+    cpu_s = v_bin / target ** 2 / cpu["desorptions_per_s"]
+    return {"tally": "fraction of desorbed particles absorbed by the 8 pumps", "p": p,
+            "sigma_target": target, "gpu_s": gpu_s, "cpu_s": cpu_s, "speedup": cpu_s / gpu_s,
+            "sigma_per_sqrt_desorption": {"binomial": math.sqrt(v_bin), "gpu_batch_means": math.sqrt(v_bm),
+                                          "batches": len(fr), "desorptions_per_batch": n},
+            "cpu_cores": cpu["cores"]}
 
 
 def progress(msg):
@@ -308,18 +418,17 @@ def progress(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def batched_bench(args):
+def batched_leg(n_variants, lo, hi, graph=False):
     """Uneven variants of the beam pipe (facet counts log-uniform in 2k-150k,
     NEG sticking log-uniform in 1e-4..1e-1, seeded per variant): one
-    persistent launch over all of them vs one launch per variant, each
-    tracing its desorptions to the end. Setup (model upload, BVH build) is
-    outside both timed regions."""
+    persistent launch over all of them (optionally replayed from a captured
+    hipGraph) vs one launch per variant, each tracing its desorptions to
+    the end. Setup (model upload, BVH build) is outside both timed regions."""
     import numpy as np
 
     from molflow_sources_amd import geometry
     from molflow_sources_amd.simulation import GpuSimulation
-    lo, hi = (float(x) for x in args.variant_des.split(","))
-    vs = geometry.variant_models(args.variants, seed=42)
+    vs = geometry.variant_models(n_variants, seed=42)
     progress(f"{len(vs)} variants, {sum(m.n_facets for m in vs)} facets")
     rng = np.random.default_rng(7)
     n_des = np.exp(rng.uniform(math.log(lo), math.log(hi), len(vs))).astype(np.uint64)
@@ -332,8 +441,11 @@ def batched_bench(args):
     progress("batched accel built")
     tr = sim.autotune(int(n_des.sum() // 8))
     progress(f"autotune: {tr.waves} waves/SIMD, block {tr.block}")
+    if graph:
+        sim.enable_graph(True)
     sim.reset()
-    sim.run_variants(np.minimum(n_des, 1000), seeds, max_events=max_events)   # warm-up
+    sim.run_variants(n_des, seeds, max_events=max_events)   # warm-up (and graph capture)
+    sim.clear_tallies()
     sim.reset()
     t0 = time.perf_counter()
     r = sim.run_variants(n_des, seeds, max_events=max_events)
@@ -365,22 +477,22 @@ def batched_bench(args):
         "per_variant_launches_hits_per_s": hits_p / wall_p, "speedup_batched": (hits_b / wall_b) / (hits_p / wall_p),
         "batched_wall_s": wall_b, "per_variant_wall_s": wall_p, "batched_kernel_ms": kms_b,
         "per_variant_kernel_ms": kms_p, "hits": hits_b, "identical_hits_per_variant": bool(same),
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f64", "data": "synthetic", "hipgraph": bool(graph),
         "config": {"workload": "beam_pipe_variants", "variants": len(vs),
                    "facets_total": int(sum(m.n_facets for m in vs)),
                    "facets_min": int(min(m.n_facets for m in vs)), "facets_max": int(max(m.n_facets for m in vs)),
                    "desorptions_total": int(n_des.sum()), "desorptions_per_variant": [lo, hi],
                    "launch": {"waves_per_simd": tr.waves, "block": tr.block, "blocks_per_cu": tr.blocks_per_cu}},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def adjoint_bench(args):
+def adjoint_leg(n, fd_des=0):
     """Sticking adjoint on the beam pipe in low-flux mode: 32 NEG sub-bands
     (base sticking log-uniform in 1e-3..5e-2), 64 target wall facets. Cost
     of a run with the path-score Jacobian vs the same forward run without
-    it, and every d(upstream . J0)/ds_g against central finite differences
-    (ds = 1e-3 s_g) with common random numbers, in particle batches."""
+    it; with fd_des > 0 also every d(upstream . J0)/ds_g against finite
+    differences of the same particles (fd_check)."""
     import numpy as np
 
     from molflow_sources_amd import geometry
@@ -393,7 +505,6 @@ def adjoint_bench(args):
         m.facets["sticking"][grp == g] = s
     up = np.random.default_rng(42).normal(size=len(tg))
     max_events = 50_000_000
-    n = args.adjoint_des
     a = GpuSimulation(0)
     a.load(m)
     a.rebuild_accel()
@@ -429,8 +540,35 @@ def adjoint_bench(args):
     tf_used = shapes[k]
     progress(f"forward: {rf.hits} hits in {rf.kernel_ms:.1f} ms (launches tried: "
              + ", ".join(f"{r.kernel_ms:.1f} ms" for r in fwd) + ")")
-    # finite differences with common random numbers, in batches for sigma
-    nb, per = 8, max(1, args.fd_des // 8)
+    out = {
+        "metric": "adjoint cost vs one forward run, beam pipe, 32 sticking groups x 64 targets",
+        "value": ra.kernel_ms / rf.kernel_ms, "unit": "x forward kernel time", "n_gpus": 1,
+        "higher_is_better": False, "forward_kernel_ms": rf.kernel_ms, "adjoint_kernel_ms": ra.kernel_ms,
+        "forward_hits_per_s": rf.hits / (rf.kernel_ms / 1e3), "adjoint_hits_per_s": ra.hits / (ra.kernel_ms / 1e3),
+        "hits": ra.hits,
+        "launch": {"forward": list(tf_used[:2]), "adjoint": [ta.waves, ta.block],
+                   "forward_kernel_ms_tried": [r.kernel_ms for r in fwd]},
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "beam_pipe_low_flux_adjoint", "facets": m.n_facets, "desorptions": n,
+                   "groups": len(base), "targets": len(tg), "low_flux_cutoff": 1e-6},
+    }
+    if fd_des > 0:
+        out["gradient"] = [float(x) for x in grad]
+        out["fd_check"] = fd_check(a, base, up, fd_des, max_events)
+    a.close()
+    return out
+
+
+def fd_check(a, base, up, fd_des, max_events, nb=8):
+    """Every d(up . J0)/ds_g against finite differences of the same runs
+    with common random numbers (the same particles, so the same paths),
+    in nb independent particle batches. The differences are Richardson-
+    extrapolated central differences at h and 2h (h = 1e-3 s_g), which
+    removes the O(h^2) truncation; what remains is the low-flux cutoff's
+    boundary term and noise. z divides the mean paired difference by its
+    own batch standard error."""
+    import numpy as np
+    per = max(1, fd_des // nb)
 
     def objective(s, b):
         a.adjoint_set_sticking(s)
@@ -442,42 +580,26 @@ def adjoint_bench(args):
     adj_b = np.array([objective(base, b)[1] for b in range(nb)])     # [batch][group]
     fd_b = np.zeros_like(adj_b)
     for g in range(len(base)):
+        h = 1e-3 * base[g]
         for b in range(nb):
-            ds = 1e-3 * base[g]
-            sp, sm = base.copy(), base.copy()
-            sp[g] += ds
-            sm[g] -= ds
-            fp, _ = objective(sp, b)
-            fm, _ = objective(sm, b)
-            fd_b[b, g] = (fp - fm) / (2 * ds)
+            f = {}
+            for k in (-2, -1, 1, 2):
+                sv = base.copy()
+                sv[g] += k * h
+                f[k] = objective(sv, b)[0]
+            d1 = (f[1] - f[-1]) / (2 * h)
+            d2 = (f[2] - f[-2]) / (4 * h)
+            fd_b[b, g] = (4 * d1 - d2) / 3
         progress(f"finite differences: group {g + 1}/{len(base)}")
-    a.close()
-    # sigma of each sensitivity: standard error of the batch means of the
-    # adjoint estimate (independent particle blocks)
-    sem = adj_b.std(axis=0, ddof=1) / math.sqrt(nb)
-    z = (fd_b.mean(0) - adj_b.mean(0)) / np.where(sem > 0, sem, np.inf)
     d = fd_b - adj_b
     sem_d = d.std(axis=0, ddof=1) / math.sqrt(nb)
-    z_crn = np.where(sem_d > 0, d.mean(0) / np.where(sem_d > 0, sem_d, 1.0), 0.0)
-    out = {
-        "metric": "adjoint cost vs one forward run, beam pipe, 32 sticking groups x 64 targets",
-        "value": ra.kernel_ms / rf.kernel_ms, "unit": "x forward kernel time", "n_gpus": 1,
-        "higher_is_better": False, "forward_kernel_ms": rf.kernel_ms, "adjoint_kernel_ms": ra.kernel_ms,
-        "forward_hits_per_s": rf.hits / (rf.kernel_ms / 1e3), "adjoint_hits_per_s": ra.hits / (ra.kernel_ms / 1e3),
-        "hits": ra.hits, "gradient": [float(x) for x in grad],
-        "fd_check": {"groups_within_2sigma": int(np.sum(np.abs(z) <= 2)), "groups": len(z),
-                     "max_abs_z": float(np.max(np.abs(z))), "z": [float(x) for x in z],
-                     "sigma": "standard error of the sensitivity over independent particle batches",
-                     "max_rel_diff": float(np.max(np.abs(fd_b.mean(0) - adj_b.mean(0)) / np.abs(adj_b.mean(0)))),
-                     "crn_difference_z": [float(x) for x in z_crn],
-                     "batches": nb, "desorptions_per_batch": per},
-        "launch": {"forward": list(tf_used[:2]), "adjoint": [ta.waves, ta.block],
-                   "forward_kernel_ms_tried": [r.kernel_ms for r in fwd]},
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "beam_pipe_low_flux_adjoint", "facets": m.n_facets, "desorptions": n,
-                   "groups": len(base), "targets": len(tg), "low_flux_cutoff": 1e-6},
-    }
-    print(json.dumps(out), flush=True)
+    z = np.where(sem_d > 0, d.mean(0) / np.where(sem_d > 0, sem_d, 1.0), 0.0)
+    rel = np.abs(d.mean(0)) / np.abs(adj_b.mean(0))
+    return {"paired_z": [float(x) for x in z], "max_abs_z": float(np.max(np.abs(z))),
+            "groups_within_2": int(np.sum(np.abs(z) <= 2)), "groups": len(z),
+            "mean_z_squared": float(np.mean(z ** 2)), "max_rel_diff": float(np.max(rel)),
+            "difference": "Richardson central differences (h, 2h; h = 1e-3 s_g) minus the path-score "
+                          "gradient, same particles", "batches": nb, "desorptions_per_batch": per}
 
 
 if __name__ == "__main__":

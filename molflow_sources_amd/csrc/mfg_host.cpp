@@ -13,6 +13,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -884,6 +885,8 @@ struct mfg_context {
     uint32_t adj_groups = 0, adj_targets = 0;
     DevBuf<double> adj_j0, adj_jac, adj_s, adj_up, adj_grad;
     DevBuf<uint32_t> adj_counts;
+    DevBuf<int32_t> adj_seg_t;           // per lane: target scores of the current segment (cutoff term)
+    DevBuf<double> adj_seg_x;
     std::vector<int32_t> adj_group_of, adj_target_list;
     uint64_t adj_des = 0;
     bool adj_s_ok = false;               // every group sticking inside (0,1)
@@ -1082,7 +1085,7 @@ void upload_models(mfg_context* c) {
     c->prof.alloc(std::max<uint64_t>(prof, 1)); c->am.alloc(std::max<uint64_t>(am, 1));
     c->gh.alloc(std::max<uint64_t>(gh, 1)); c->fh.alloc(std::max<uint64_t>(fh, 1));
     c->glob.alloc(c->models.size());
-    c->counters.alloc(8);
+    c->counters.alloc(9);   // [8]: adjoint segment overflow flag
     c->log_cap = log_cap;
     c->log.alloc(std::max<uint32_t>(log_cap, 1));
     c->leaks.alloc((size_t)kLeakCap * 6);
@@ -1153,6 +1156,15 @@ void breadth_first(std::vector<BvhNode>& nodes, std::vector<int32_t>& roots) {
     nodes.swap(out);
     for (int32_t& r : roots)
         if (r >= 0) r = idx[(size_t)r];
+}
+
+// next-node load before the leaf tests (MFG_NODE_PREFETCH=1 turns it on)
+uint32_t node_prefetch_on() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("MFG_NODE_PREFETCH");
+        return (uint32_t)(e && std::atoi(e) == 1 ? 1 : 0);
+    }();
+    return v;
 }
 
 void build_accel(mfg_context* c, int width, int leaf) {
@@ -1259,6 +1271,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     TraceParams p{};
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
     p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
+    p.node_prefetch = node_prefetch_on();
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size();
     p.stack_size = std::min<uint32_t>(c->stack_size, MFG_LDS_STACK);
     p.stack_total = c->stack_size;
@@ -1282,6 +1295,7 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.lds_nodes = lds_node_count(c);
     p.trace_facet = c->trace_f.p; p.trace_kind = c->trace_kind.p; p.trace_n = c->trace_n; p.trace_k = c->trace_k;
     p.adj_j0 = c->adj_j0.p; p.adj_jac = c->adj_jac.p; p.adj_sticking = c->adj_s.p; p.adj_counts = c->adj_counts.p;
+    p.adj_seg_t = c->adj_seg_t.p; p.adj_seg_x = c->adj_seg_x.p; p.adj_seg_overflow = c->counters.p + 8;
     p.adj_groups = c->adj_groups; p.adj_targets = c->adj_targets;
     return p;
 }
@@ -1350,6 +1364,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
             throw Err(MFG_ERR_STATE, "a sticking group's value is not inside (0,1): set it with mfg_adjoint_set_sticking");
         size_t need = (size_t)grid * block * c->adj_groups;
         if (c->adj_counts.n < need) c->adj_counts.alloc(need);
+        const size_t seg = (size_t)grid * block * MFG_ADJ_SEG;
+        if (c->adj_seg_t.n < seg) { c->adj_seg_t.alloc(seg); c->adj_seg_x.alloc(seg); reset_graph(c); }
     }
     if (c->carry_on) ensure_carry(c, (uint64_t)grid * (uint64_t)block);
     if (c->stack_size > MFG_LDS_STACK) {
@@ -1374,7 +1390,8 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
         HIPCHK(hipMemcpyAsync(c->variants.p, c->pin_vars, nv * sizeof(DVariant), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(c->prefix.p, c->pin_prefix, (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
         if (mfg_launch_zero(c->counters.p, sizeof(unsigned long long), s) ||
-            mfg_launch_zero(c->counters.p + 3, sizeof(unsigned long long), s))
+            mfg_launch_zero(c->counters.p + 3, sizeof(unsigned long long), s) ||
+            (c->adj_groups && mfg_launch_zero(c->counters.p + 8, sizeof(unsigned long long), s)))
             throw Err(MFG_ERR_DEVICE, "zero kernel launch failed");
         if (events) HIPCHK(hipEventRecord(c->ev0, s));
         if (mfg_launch_trace(p, feats, waves, grid, block, s)) throw Err(MFG_ERR_DEVICE, std::string("trace launch failed: ") + hipGetErrorString(hipGetLastError()));
@@ -1415,7 +1432,13 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
         des += after[v].nb_desorbed - before[v].nb_desorbed;
         c->pid_base[v] += n_des[v];
     }
-    if (c->adj_groups) c->adj_des += des;
+    if (c->adj_groups) {
+        unsigned long long ov = 0;
+        HIPCHK(hipMemcpy(&ov, c->counters.p + 8, sizeof(ov), hipMemcpyDeviceToHost));
+        if (ov) throw Err(MFG_ERR_LIMIT, "adjoint: a particle scored on targets more than " + std::to_string(MFG_ADJ_SEG) +
+                                             " times between two sticky collisions (cutoff term incomplete)");
+        c->adj_des += des;
+    }
     if (res) {
         res->status = 0;
         res->stopped = expired ? 2 : *c->stop_host ? 1 : 0;
@@ -1574,6 +1597,7 @@ void launch_batch(mfg_context* c, uint64_t n, const double* dO, const double* dD
                   int32_t* dF, double* dT, double* dU, double* dV) {
     IsectParams p{};
     p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p;
+    p.node_prefetch = node_prefetch_on();
     p.root = c->h_roots.at(c->dvars[0].root_base + (uint32_t)structure);
     p.stack_size = c->stack_size; p.n = n; p.origins = dO; p.dirs = dD; p.last_hit = dL;
     p.out_facet = dF; p.out_t = dT; p.out_u = dU; p.out_v = dV; p.facet_base = c->dvars[0].facet_base;

@@ -17,6 +17,7 @@
 
 #include "../../include/molflow_gpu.h"
 
+#define MFG_ADJ_SEG 32            // target scores kept per lane for the adjoint's cutoff term
 #define MFG_BVH_WIDTH 8          // children per node (a 4-wide build uses 4 of the 8 slots)
 #define MFG_MAX_STACK 64
 #ifndef MFG_LDS_STACK
@@ -231,6 +232,7 @@ struct TraceParams {
     const BvhNode* nodes;
     const LeafFacet* leaves;
     const LeafUV* leaf_uv;
+    uint32_t node_prefetch;           // 1: load the next node before this node's leaf tests
     const int32_t* roots;             // per (variant, structure): root node or -1
     const DVariant* variants;
     const uint64_t* var_prefix;       // n_variants+1 prefix of this run's desorptions
@@ -301,6 +303,12 @@ struct TraceParams {
     double* adj_jac;
     const double* adj_sticking;
     uint32_t* adj_counts;             // per lane: sticky collisions per group
+    // per lane: the target scores recorded since the particle last survived
+    // a sticky collision (MFG_ADJ_SEG entries), for the cutoff term; a lane
+    // with more sets *adj_seg_overflow
+    int32_t* adj_seg_t;
+    double* adj_seg_x;
+    unsigned long long* adj_seg_overflow;
     uint32_t adj_groups, adj_targets;
 };
 
@@ -310,6 +318,7 @@ struct IsectParams {
     const BvhNode* nodes;
     const LeafFacet* leaves;
     const LeafUV* leaf_uv;
+    uint32_t node_prefetch;
     int32_t root;
     uint32_t stack_size;
     uint64_t n;

@@ -576,6 +576,58 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     float tbf = widen(tmax);
     uint32_t sp = 0;
     int32_t node = root;
+    if (P.node_prefetch) {
+        // the next node's load is issued before this node's leaf tests, so
+        // that its latency and theirs overlap; children are pushed with the
+        // bound known before those tests (a node popped beyond a nearer hit
+        // found by them culls its own children)
+        NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
+        for (;;) {
+            float tn[8];
+            const uint32_t m = node_hits(nd, r, tbf, tn);
+            const uint32_t leaves = nd.c6.z & 0xffu;
+            uint32_t lm = m & leaves;
+            const uint32_t im = m & ~leaves;
+            const uint4 w1 = nd.c1, w2 = nd.c2;
+            const uint32_t k0 = nd.c3.x, k1 = nd.c3.y;
+#pragma unroll
+            for (int c = 7; c >= 0; --c)
+                if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
+            const bool more = sp != 0;
+            if (more) {
+                node = S.pop(sp);
+                nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
+            }
+            while (lm) {
+                const uint32_t c = (uint32_t)__builtin_ctz(lm);
+                lm &= lm - 1u;
+                const uint32_t wd = c == 0 ? w1.x : c == 1 ? w1.y : c == 2 ? w1.z : c == 3 ? w1.w
+                                  : c == 4 ? w2.x : c == 5 ? w2.y : c == 6 ? w2.z : w2.w;
+                const uint32_t first = ~wd, cnt = ((c < 4 ? k0 : k1) >> (8u * (c & 3u))) & 0xffu;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    double t, u, v;
+                    int32_t f;
+                    uint32_t fl;
+                    bool hit = leaf_test(P.leaves + first + k, P.leaf_uv + first + k, P.poly, Pp, D, skip, t, u, v, f, fl);
+                    if (!hit) continue;
+                    if (!(t < h.t || (t == h.t && f < h.facet))) continue;
+                    bool hard = true;
+                    if constexpr (TRANSP) {
+                        if (!opaque_only && !(fl & FF_OPAQUE)) {
+                            double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
+                            if (op < 1.0) hard = (op > 0.0) && rng->coin(event, (uint32_t)(f - (int32_t)var->facet_base)) < op;
+                        }
+                    }
+                    if (hard) {
+                        h.facet = f; h.t = t; h.u = u; h.v = v;
+                        tbf = widen(t);
+                    }
+                }
+            }
+            if (!more) break;
+        }
+        return h;
+    }
     for (;;) {
         const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
         float tn[8];
@@ -643,6 +695,9 @@ struct Tracer {
     uint32_t acc_leak, acc_decay;
     uint32_t tr_n;
     uint32_t* adj_cnt;
+    uint32_t seg_n;                   // adjoint: target scores since the last survived sticky collision
+    int32_t* seg_t;
+    double* seg_x;
     double* hot;                      // LDS table of the hot facets' slot-0 counters (8 words each)
     const uint4* lnodes;              // LDS copy of the first P.lds_nodes BVH nodes
     // pending slot-0 facet counter update of this lane
@@ -656,6 +711,9 @@ struct Tracer {
         acc_leak = acc_decay = 0;
         tr_n = 0;
         adj_cnt = nullptr;
+        seg_n = 0;
+        seg_t = nullptr;
+        seg_x = nullptr;
         var = nullptr;
         hot = nullptr;
         lnodes = nullptr;
@@ -810,8 +868,37 @@ struct Tracer {
                 if (absorbed_part && ph.adj_group == (int)g) d += x / s;
                 if (d != 0.0) atomic_addd(P.adj_jac + (size_t)t * G + g, d);
             }
+            // kept for the cutoff term until the particle survives a sticky collision
+            if (seg_n < MFG_ADJ_SEG) { seg_t[seg_n] = t; seg_x[seg_n] = x; }
+            ++seg_n;
         }
         (void)ph; (void)x; (void)absorbed_part;
+    }
+    // The low-flux cutoff's term of dE[J]/ds_g (not a derivative of any
+    // single path): the particle ends at collision K, W_{K-1} (1 - s_K) <=
+    // c < W_{K-1}; a rise ds of s_g ends it at K-1 instead with probability
+    // f(0) n_g ds / (1 - s_g) (n_g: its collisions with group g up to K-1,
+    // f(0): density of ln(W_{K-1}/c) at 0), losing the scores recorded since
+    // it survived K-1. In the renewal limit ln(W_{K-1}/c) is uniform on
+    // (0, -ln(1 - s_K)] given K, so the term is
+    //   dJ_t/ds_g += -sum x_t n_g / ((1 - s_g)(-ln(1 - s_K))).
+    __device__ void cutoff_boundary(const DPhys& ph, double sK) {
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            if (seg_n > MFG_ADJ_SEG) atomicOr(P.adj_seg_overflow, 1ull);
+            const double dK = -log1p(-sK);
+            const uint32_t n_seg = seg_n < MFG_ADJ_SEG ? seg_n : MFG_ADJ_SEG;
+            if (!(dK > 0.0) || n_seg == 0) return;
+            const uint32_t G = P.adj_groups;
+            for (uint32_t i = 0; i < n_seg; ++i) {
+                const int t = seg_t[i];
+                const double x = seg_x[i];
+                for (uint32_t g = 0; g < G; ++g) {
+                    const double n = (double)adj_cnt[g] - (ph.adj_group == (int)g ? 1.0 : 0.0);
+                    if (n > 0.0) atomic_addd(P.adj_jac + (size_t)t * G + g, -x * n / ((1.0 - P.adj_sticking[g]) * dK));
+                }
+            }
+        }
+        (void)ph; (void)sK;
     }
     __device__ void counter_force(const DPhys& ph, int m, uint32_t hit, uint32_t des, uint32_t abs_, double s1v,
                                   double svo, double sign) {
@@ -1035,7 +1122,10 @@ struct Tracer {
         rng.init(var->seed, pid);
         nb_events = 0;
         tr_n = 0;
-        if constexpr ((FEAT & FEAT_ADJOINT) != 0) { for (uint32_t g = 0; g < P.adj_groups; ++g) adj_cnt[g] = 0; }
+        if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
+            for (uint32_t g = 0; g < P.adj_groups; ++g) adj_cnt[g] = 0;
+            seg_n = 0;
+        }
         double r = rng.rnd() * var->total_des;
         const double* cum = P.src_cum + var->src_base;
         uint32_t n = var->src_n;
@@ -1487,7 +1577,12 @@ struct Tracer {
             } else {
                 ori *= (1.0 - s);
             }
-            if (ori > var->low_flux_cutoff) { bounce(f); return true; }
+            if (ori > var->low_flux_cutoff) {
+                if constexpr ((FEAT & FEAT_ADJOINT) != 0) { if (s > 0.0) seg_n = 0; }
+                bounce(f);
+                return true;
+            }
+            if constexpr ((FEAT & FEAT_ADJOINT) != 0) { if (s > 0.0) cutoff_boundary(ph, s); }
             trace(f, 2);
             return false;
         }
@@ -1535,6 +1630,8 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     T.lnodes = lnodes;
     if constexpr ((FEAT & FEAT_ADJOINT) != 0) {
         T.adj_cnt = P.adj_counts + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * P.adj_groups;
+        T.seg_t = P.adj_seg_t + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * MFG_ADJ_SEG;
+        T.seg_x = P.adj_seg_x + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * MFG_ADJ_SEG;
     }
     bool alive = false, done = false;
     uint32_t vi = 0;
@@ -1782,6 +1879,7 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     if (i >= p.n) return;
     TraceParams tp{};
     tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaves = p.leaves; tp.leaf_uv = p.leaf_uv;
+    tp.node_prefetch = p.node_prefetch;
     d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
     int32_t skip = p.last_hit ? p.last_hit[i] : -1;
     if (skip >= 0) skip += (int32_t)p.facet_base;

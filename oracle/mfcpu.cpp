@@ -975,6 +975,7 @@ struct Tracer {
     Rng rng;
     Adjoint* adj = nullptr;
     std::vector<uint32_t> ncoll;  // adjoint: sticky collisions per group
+    std::vector<std::pair<int, double>> seg;   // adjoint: (target, score) since the last survived collision
     // particle state (Particle.h:95-122)
     V3 pos{}, dir{};
     double time = 0, velocity = 0, ori = 1, gen_time = 0, dist = 0, decay_t = 1e100, scatter_path = 1e100;
@@ -1017,6 +1018,8 @@ struct Tracer {
     }
     // Path-score contribution of a hit-equivalent x recorded on target fid:
     // d x / d s_g = x * (-n_g / (1 - s_g)) (+ x / s_t for the absorbed part).
+    // The score is also kept in `seg` (scores since the last collision the
+    // particle survived) for the cutoff term of cutoff_boundary.
     void adjoint_score(int fid, double x, bool absorbed_part) {
         int t = adj->target_of[fid];
         adj->j0[t] += x;
@@ -1025,6 +1028,28 @@ struct Tracer {
             if (absorbed_part && adj->group[fid] == (int)g) d += x / adj->s[g];
             adj->jac[(size_t)t * adj->G + g] += d;
         }
+        seg.push_back({t, x});
+    }
+    // The low-flux cutoff's term of d E[J] / d s_g (not a derivative of any
+    // single path). A particle ends at the first collision K whose weight
+    // falls to the cutoff c: W_K = W_{K-1} (1 - s_K) <= c < W_{K-1}. A rise
+    // ds of s_g lowers W_{K-1} by W_{K-1} n_g ds / (1 - s_g) (n_g: the
+    // particle's collisions with group g up to K-1), so with probability
+    // f(0) n_g ds / (1 - s_g) the particle ends at K-1 instead, losing the
+    // scores recorded after it survived K-1 (`seg`); f(0) is the density of
+    // ln(W_{K-1} / c) at 0. In the renewal limit (ln(1/c) spans many
+    // collisions) ln(W_{K-1} / c) is uniform on (0, -ln(1 - s_K)] given the
+    // collision K that ends the particle, so f(0) = 1 / -ln(1 - s_K):
+    //   dJ_t / ds_g += -sum_seg x_t n_g / ((1 - s_g) (-ln(1 - s_K))).
+    void cutoff_boundary(int fid, double sK) {
+        const double dK = -std::log1p(-sK);
+        if (!(dK > 0.0)) return;
+        const int gK = adj->group[fid];
+        for (const auto& e : seg)
+            for (uint32_t g = 0; g < adj->G; ++g) {
+                const double n = (double)ncoll[g] - (gK == (int)g ? 1.0 : 0.0);
+                if (n > 0.0) adj->jac[(size_t)e.first * adj->G + g] -= e.second * n / ((1.0 - adj->s[g]) * dK);
+            }
     }
     void force_terms(V3 vel, V3& v2, V3& mom) {
         v2 = {vel.x * vel.x, vel.y * vel.y, vel.z * vel.z};
@@ -1398,6 +1423,7 @@ struct Tracer {
         rng.init(seed, pid);
         nb_events = 0;
         if (adj) std::fill(ncoll.begin(), ncoll.end(), 0u);
+        seg.clear();
         if (!start_from_source()) return;
         for (;;) {
             if (max_events && nb_events >= max_events) return;
@@ -1437,8 +1463,14 @@ struct Tracer {
                         ori = w0 * (1.0 - s);
                         if (adj && adj->group[h.facet] >= 0) ncoll[adj->group[h.facet]]++;
                     } else ori *= (1.0 - s);
-                    if (ori > M.s.low_flux_cutoff) perform_bounce(h.facet);
-                    else { trace(h.facet, 2); return; }
+                    if (ori > M.s.low_flux_cutoff) {
+                        if (s > 0.0) seg.clear();
+                        perform_bounce(h.facet);
+                    } else {
+                        if (adj && s > 0.0) cutoff_boundary(h.facet, s);
+                        trace(h.facet, 2);
+                        return;
+                    }
                 }
             } else if (ev == 1 || ev == 2) {
                 T.g.dist_traveled_total += travel;

@@ -48,32 +48,44 @@ def test_autograd_gradient_equals_path_score_contraction(unit):
 
 
 def test_autograd_matches_finite_difference_of_its_forward(unit):
-    """The torch gradient of a scalar loss against central differences of
-    the same forward with common random numbers (same seed, same paths)."""
+    """The torch gradient of a scalar loss against finite differences of the
+    same forward with common random numbers (same seed, same paths):
+    Richardson-extrapolated central differences at h and 2h (h = 1e-2 s_g)
+    remove the O(h^2) term; over 8 seeds the mean of the paired difference
+    lies within 3.5 of its own standard errors. The gradient carries the
+    low-flux cutoff's term; without it the differences show the cutoff's
+    bias."""
     from molflow_sources_amd.adjoint import sticking_response
     sim, base, tg = unit
-    n = 20_000
+    n, nb = 10_000, 8
     w = torch.tensor(np.random.default_rng(4).normal(size=len(tg)))
 
-    def loss(sv, grad=False):
+    def loss(sv, seed, grad=False):
         s = torch.tensor(sv, dtype=torch.float64, requires_grad=grad)
-        L = (sticking_response(s, sim, n, seed=23, max_events=MAXEV) * w).sum()
+        L = (sticking_response(s, sim, n, seed=seed, max_events=MAXEV) * w).sum()
         if grad:
             L.backward()
             return float(L.detach()), s.grad.detach().numpy().copy()
         return float(L.detach())
 
-    _, g = loss(base, grad=True)
     for k in (0, len(base) // 2, len(base) - 1):
-        ds = 1e-3 * base[k]
-        sp, sm = base.copy(), base.copy()
-        sp[k] += ds
-        sm[k] -= ds
-        fd = (loss(sp) - loss(sm)) / (2 * ds)
-        # common random numbers: the paths are shared, only weights and the
-        # low-flux cutoff point move; that cutoff leaves an absolute error
-        # of order 1e-5 of the gradient's scale
-        assert abs(fd - g[k]) <= 2e-3 * abs(g[k]) + 1e-5 * np.abs(g).max(), (k, fd, g[k])
+        h = 1e-2 * base[k]
+        d = []
+        for b in range(nb):
+            seed = 100 + b
+            _, g = loss(base, seed, grad=True)
+            f = {}
+            for q in (-2, -1, 1, 2):
+                sv = base.copy()
+                sv[k] += q * h
+                f[q] = loss(sv, seed)
+            d1 = (f[1] - f[-1]) / (2 * h)
+            d2 = (f[2] - f[-2]) / (4 * h)
+            d.append((4 * d1 - d2) / 3 - g[k])
+        print(f"paired differences: group {k} done", flush=True)
+        d = np.array(d)
+        sem = d.std(ddof=1) / np.sqrt(nb)
+        assert abs(d.mean()) <= 3.5 * sem + 1e-12 * np.abs(g).max(), (k, d.mean(), sem)
 
 
 def test_backward_after_another_run_is_refused(unit):

@@ -28,6 +28,7 @@ from scipy import stats
 import feature_models as FM
 from molflow_sources_amd import _abi as A
 from molflow_sources_amd import geometry
+from molflow_sources_amd.model import default_settings
 from oracle.mfcpu import CpuModel
 
 N = 60_000
@@ -344,3 +345,46 @@ def test_teleport_back_returns_to_the_last_destination(oracle_lib):
                 seen += 1
                 assert f[row, j + 1] not in box1, (row, j, f[row, : j + 2], kd[row, : j + 2])
     assert seen > 100
+
+
+def test_adjoint_cutoff_term_matches_paired_differences(oracle_lib):
+    """The low-flux cutoff's term of the sticking gradient. A closed cube
+    with a transparent source and two sticky walls (groups s = 0.2 and
+    0.05), cutoff 1e-3, two targets: the gradient of up . J0 against
+    Richardson-extrapolated central differences (h and 2h, h = 1e-2 s_g) of
+    the same particles. Without the term the path derivatives miss about
+    0.7 % of the gradient here (measured with it switched off); with it the
+    difference is below 0.25 % (the renewal approximation behind the term
+    leaves a few 1e-4 for a group with only a few distinct weight steps)."""
+    from oracle.mfcpu import CpuModel
+    b = FM.cube(10.0)
+    b.add("source", FM.square(5.0, 5.0, 5.0, 0.5), [0, 0, -1], desorb_type=A.DES_COSINE, outgassing=1.0,
+          opacity=0.0)
+    s = default_settings()
+    s.low_flux_mode = 1
+    s.low_flux_cutoff = 1e-3
+    m = b.model(s)
+    grp = np.full(m.n_facets, -1, dtype=np.int32)
+    grp[m.ids["xlo"]] = 0
+    grp[m.ids["ylo"]] = 1
+    base = np.array([0.2, 0.05])
+    for k, v in enumerate(base):
+        m.facets["sticking"][grp == k] = v
+    tg = np.array([m.ids["zhi"], m.ids["xhi"]], dtype=np.int32)
+    cm = CpuModel(m)
+    up = np.array([1.0, 0.7])
+    per, nb = 20_000, 8
+
+    def objective(sv, blk):
+        j0, jac = cm.adjoint(per, grp, sv, tg, seed=9, first=blk * per)
+        return float(up @ j0) / per, (up @ jac) / per
+
+    adj = np.array([objective(base, blk)[1] for blk in range(nb)])
+    fd = np.zeros_like(adj)
+    for g in range(2):
+        h = 1e-2 * base[g]
+        for blk in range(nb):
+            f = {q: objective(base + q * h * np.eye(2)[g], blk)[0] for q in (-2, -1, 1, 2)}
+            fd[blk, g] = (4 * (f[1] - f[-1]) / (2 * h) - (f[2] - f[-2]) / (4 * h)) / 3
+    rel = (fd - adj).mean(0) / np.abs(adj.mean(0))
+    assert np.all(np.abs(rel) < 2.5e-3), rel
