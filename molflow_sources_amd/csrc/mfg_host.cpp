@@ -1158,15 +1158,6 @@ void breadth_first(std::vector<BvhNode>& nodes, std::vector<int32_t>& roots) {
         if (r >= 0) r = idx[(size_t)r];
 }
 
-// next-node load before the leaf tests (MFG_NODE_PREFETCH=1 turns it on)
-uint32_t node_prefetch_on() {
-    static const uint32_t v = [] {
-        const char* e = std::getenv("MFG_NODE_PREFETCH");
-        return (uint32_t)(e && std::atoi(e) == 1 ? 1 : 0);
-    }();
-    return v;
-}
-
 void build_accel(mfg_context* c, int width, int leaf) {
     if (!c->loaded) throw Err(MFG_ERR_STATE, "no model loaded");
     if (width == 0) width = MFG_BVH_WIDTH;
@@ -1271,7 +1262,6 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     TraceParams p{};
     p.facets = c->facets.p; p.phys = c->phys.p; p.poly = c->poly.p; p.nodes = c->nodes.p;
     p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p; p.roots = c->roots.p; p.variants = c->variants.p;
-    p.node_prefetch = node_prefetch_on();
     p.var_prefix = c->prefix.p; p.n_variants = (uint32_t)c->dvars.size();
     p.stack_size = std::min<uint32_t>(c->stack_size, MFG_LDS_STACK);
     p.stack_total = c->stack_size;
@@ -1597,7 +1587,6 @@ void launch_batch(mfg_context* c, uint64_t n, const double* dO, const double* dD
                   int32_t* dF, double* dT, double* dU, double* dV) {
     IsectParams p{};
     p.facets = c->facets.p; p.poly = c->poly.p; p.nodes = c->nodes.p; p.leaves = c->leaves.p; p.leaf_uv = c->leaf_uv.p;
-    p.node_prefetch = node_prefetch_on();
     p.root = c->h_roots.at(c->dvars[0].root_base + (uint32_t)structure);
     p.stack_size = c->stack_size; p.n = n; p.origins = dO; p.dirs = dD; p.last_hit = dL;
     p.out_facet = dF; p.out_t = dT; p.out_u = dU; p.out_v = dV; p.facet_base = c->dvars[0].facet_base;

@@ -232,7 +232,6 @@ struct TraceParams {
     const BvhNode* nodes;
     const LeafFacet* leaves;
     const LeafUV* leaf_uv;
-    uint32_t node_prefetch;           // 1: load the next node before this node's leaf tests
     const int32_t* roots;             // per (variant, structure): root node or -1
     const DVariant* variants;
     const uint64_t* var_prefix;       // n_variants+1 prefix of this run's desorptions
@@ -318,7 +317,6 @@ struct IsectParams {
     const BvhNode* nodes;
     const LeafFacet* leaves;
     const LeafUV* leaf_uv;
-    uint32_t node_prefetch;
     int32_t root;
     uint32_t stack_size;
     uint64_t n;

@@ -576,58 +576,6 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     float tbf = widen(tmax);
     uint32_t sp = 0;
     int32_t node = root;
-    if (P.node_prefetch) {
-        // the next node's load is issued before this node's leaf tests, so
-        // that its latency and theirs overlap; children are pushed with the
-        // bound known before those tests (a node popped beyond a nearer hit
-        // found by them culls its own children)
-        NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
-        for (;;) {
-            float tn[8];
-            const uint32_t m = node_hits(nd, r, tbf, tn);
-            const uint32_t leaves = nd.c6.z & 0xffu;
-            uint32_t lm = m & leaves;
-            const uint32_t im = m & ~leaves;
-            const uint4 w1 = nd.c1, w2 = nd.c2;
-            const uint32_t k0 = nd.c3.x, k1 = nd.c3.y;
-#pragma unroll
-            for (int c = 7; c >= 0; --c)
-                if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
-            const bool more = sp != 0;
-            if (more) {
-                node = S.pop(sp);
-                nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
-            }
-            while (lm) {
-                const uint32_t c = (uint32_t)__builtin_ctz(lm);
-                lm &= lm - 1u;
-                const uint32_t wd = c == 0 ? w1.x : c == 1 ? w1.y : c == 2 ? w1.z : c == 3 ? w1.w
-                                  : c == 4 ? w2.x : c == 5 ? w2.y : c == 6 ? w2.z : w2.w;
-                const uint32_t first = ~wd, cnt = ((c < 4 ? k0 : k1) >> (8u * (c & 3u))) & 0xffu;
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    double t, u, v;
-                    int32_t f;
-                    uint32_t fl;
-                    bool hit = leaf_test(P.leaves + first + k, P.leaf_uv + first + k, P.poly, Pp, D, skip, t, u, v, f, fl);
-                    if (!hit) continue;
-                    if (!(t < h.t || (t == h.t && f < h.facet))) continue;
-                    bool hard = true;
-                    if constexpr (TRANSP) {
-                        if (!opaque_only && !(fl & FF_OPAQUE)) {
-                            double op = (fl & FF_TD_OPACITY) ? td_opacity(P, var, f, time) : (double)P.phys[f].opacity;
-                            if (op < 1.0) hard = (op > 0.0) && rng->coin(event, (uint32_t)(f - (int32_t)var->facet_base)) < op;
-                        }
-                    }
-                    if (hard) {
-                        h.facet = f; h.t = t; h.u = u; h.v = v;
-                        tbf = widen(t);
-                    }
-                }
-            }
-            if (!more) break;
-        }
-        return h;
-    }
     for (;;) {
         const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
         float tn[8];
@@ -1879,7 +1827,6 @@ __global__ void __launch_bounds__(256) k_intersect(IsectParams p) {
     if (i >= p.n) return;
     TraceParams tp{};
     tp.facets = p.facets; tp.poly = p.poly; tp.nodes = p.nodes; tp.leaves = p.leaves; tp.leaf_uv = p.leaf_uv;
-    tp.node_prefetch = p.node_prefetch;
     d3 P = ld3(p.origins + 3 * i), D = ld3(p.dirs + 3 * i);
     int32_t skip = p.last_hit ? p.last_hit[i] : -1;
     if (skip >= 0) skip += (int32_t)p.facet_base;
