@@ -328,10 +328,14 @@ __device__ __forceinline__ NodeRegs load_node_at(const BvhNode* __restrict__ nod
 }
 
 __device__ __forceinline__ uint32_t node_valid(const NodeRegs& n) { return n.c0.w >> 24; }
+// Child word of slot c (0-7) by a select tree on c's bits: 7 selects for a
+// lane-varying c (a chain of comparisons compiles to branches instead)
 __device__ __forceinline__ int32_t node_child(const NodeRegs& n, uint32_t c) {
-    uint32_t w = c == 0 ? n.c1.x : c == 1 ? n.c1.y : c == 2 ? n.c1.z : c == 3 ? n.c1.w
-               : c == 4 ? n.c2.x : c == 5 ? n.c2.y : c == 6 ? n.c2.z : n.c2.w;
-    return (int32_t)w;
+    const bool b2 = (c & 4u) != 0, b1 = (c & 2u) != 0, b0 = (c & 1u) != 0;
+    const uint32_t x = b2 ? n.c2.x : n.c1.x, y = b2 ? n.c2.y : n.c1.y;
+    const uint32_t z = b2 ? n.c2.z : n.c1.z, w = b2 ? n.c2.w : n.c1.w;
+    const uint32_t lo = b0 ? y : x, hi = b0 ? w : z;
+    return (int32_t)(b1 ? hi : lo);
 }
 __device__ __forceinline__ uint32_t node_count(const NodeRegs& n, uint32_t c) {
     return ((c < 4 ? n.c3.x : n.c3.y) >> (8u * (c & 3u))) & 0xffu;
@@ -345,7 +349,10 @@ __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) 
 // constants per axis, then one fma per bound. The quantised boxes enclose
 // the FP64 boxes padded by 1e-5 of the model extent, far above the FP32
 // rounding of these steps, so the test never drops a box that holds an
-// FP64 hit.
+// FP64 hit: a ray through a point of the unpadded box spends at least
+// 1e-5 extent / |d| of its parameter inside the padded one (within
+// [0, tmax] as well, since tmax is at or beyond that hit), against FP32
+// errors near 1e-7 extent / |d|, so no tolerance is added to a <= b.
 __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[8]) {
     const float ox = __uint_as_float(n.c0.x), oy = __uint_as_float(n.c0.y), oz = __uint_as_float(n.c0.z);
     const uint32_t ew = n.c0.w;
@@ -372,7 +379,7 @@ __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r,
         float a = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
         float b = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         tn[c] = a;
-        if (a <= b * 1.000001f) mask |= 1u << c;
+        if (a <= b) mask |= 1u << c;
     }
     return mask & node_valid(n);
 }
