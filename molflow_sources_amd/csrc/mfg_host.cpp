@@ -841,6 +841,7 @@ struct mfg_context {
     // 16 KB, which leaves 256-thread blocks their occupancy up to 5 waves
     // per SIMD
     uint32_t node_cache = 128;
+    size_t lds_per_block = 65536;        // the device's LDS limit for one block
     // device data
     DevBuf<DFacet> facets;
     DevBuf<DPhys> phys;
@@ -1253,9 +1254,18 @@ void build_accel(mfg_context* c, int width, int leaf) {
 }
 
 // Nodes read from LDS: the top of the tree of a single-variant model.
-uint32_t lds_node_count(const mfg_context* c) {
+uint32_t lds_node_count(const mfg_context* c, int block) {
     if (c->dvars.size() != 1) return 0;
-    return (uint32_t)std::min<size_t>(c->node_cache, c->nodes.n);
+    size_t n = c->node_cache;
+    if (block > 256) {
+        // a 1024-lane block serves a whole CU: the LDS its stacks and hot
+        // facets leave (at least four times a 256-lane block's copy)
+        const size_t used = (size_t)block * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
+                            c->hot.size() * 8 * sizeof(double);
+        const size_t room = c->lds_per_block > used + 2048 ? (c->lds_per_block - used - 2048) / sizeof(BvhNode) : 0;
+        n = std::max<size_t>(4 * (size_t)c->node_cache, room);
+    }
+    return (uint32_t)std::min<size_t>(n, c->nodes.n);
 }
 
 TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
@@ -1282,7 +1292,6 @@ TraceParams make_params(mfg_context* c, uint64_t total, uint64_t max_events) {
     p.expired = c->counters.p + 3;
     p.carry = c->carry.p; p.carry_ctl = c->counters.p + 4; p.carry_cap = c->carry_cap;
     p.carry_keep = c->carry_on ? 1u : 0u;
-    p.lds_nodes = lds_node_count(c);
     p.trace_facet = c->trace_f.p; p.trace_kind = c->trace_kind.p; p.trace_n = c->trace_n; p.trace_k = c->trace_k;
     p.adj_j0 = c->adj_j0.p; p.adj_jac = c->adj_jac.p; p.adj_sticking = c->adj_s.p; p.adj_counts = c->adj_counts.p;
     p.adj_seg_t = c->adj_seg_t.p; p.adj_seg_x = c->adj_seg_x.p; p.adj_seg_overflow = c->counters.p + 8;
@@ -1299,12 +1308,25 @@ int default_waves(uint32_t feats) {
 void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& waves) {
     block = (int)c->block;
     waves = c->waves ? c->waves : default_waves(feats);
+    if (block > 256) {
+        // the 1024-lane kernel exists for the lean and low-flux feature
+        // sets at 4 waves per SIMD; other runs take 256-lane blocks
+        if (mfg_trace_has_cu_block(feats)) waves = 4;
+        else block = 256;
+    }
+    auto lds_bytes = [&](int b) {
+        return (size_t)b * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
+               c->hot.size() * 8 * sizeof(double) + (size_t)lds_node_count(c, b) * sizeof(BvhNode);
+    };
+    if (block > 256 && mfg_trace_occupancy(feats, waves, block, lds_bytes(block)) < 1) {
+        block = 256;   // the block's LDS does not fit
+        waves = c->waves ? c->waves : default_waves(feats);
+    }
     uint32_t per_cu = c->blocks_per_cu;
+    if (block > 256) per_cu = 1;
     if (per_cu == 0) {
         // fill every CU to its occupancy limit (persistent launch)
-        size_t lds = (size_t)block * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
-                     c->hot.size() * 8 * sizeof(double) + (size_t)lds_node_count(c) * sizeof(BvhNode);
-        int occ = mfg_trace_occupancy(feats, waves, block, lds);
+        int occ = mfg_trace_occupancy(feats, waves, block, lds_bytes(block));
         per_cu = (uint32_t)std::max(1, occ);
     }
     grid = (int)(per_cu * (uint32_t)c->n_cu);
@@ -1368,6 +1390,7 @@ void run_variants(mfg_context* c, const uint64_t* n_des, const uint64_t* seeds, 
     HIPCHK(hipMemcpyAsync(before.data(), c->glob.p, nv * sizeof(DGlobal), hipMemcpyDeviceToHost, s));
     *c->stop_host = 0;
     TraceParams p = make_params(c, total, max_events);
+    p.lds_nodes = lds_node_count(c, block);
     std::vector<uint64_t> key(n_des, n_des + nv);
     key.push_back(max_events); key.push_back(feats); key.push_back((uint64_t)grid); key.push_back(block);
     key.push_back((uint64_t)waves);
@@ -1610,6 +1633,7 @@ int mfg_create(mfg_context** out, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MFG_ERR_DEVICE;
     c->n_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MFG_ERR_DEVICE;
     if (hipHostMalloc((void**)&c->stop_host, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return MFG_ERR_DEVICE;
     *c->stop_host = 0;
@@ -1917,7 +1941,8 @@ int mfg_get_trace(mfg_context* c, int32_t* facet, int32_t* kind, size_t n) {
 
 int mfg_set_launch(mfg_context* c, uint32_t block, uint32_t per_cu) {
     return guarded(c, [&] {
-        if (block != 0 && (block % 64 != 0 || block > 256)) throw Err(MFG_ERR_INVALID, "block must be a multiple of 64, at most 256");
+        if (block != 0 && block != 1024 && (block % 64 != 0 || block > 256))
+            throw Err(MFG_ERR_INVALID, "block must be a multiple of 64 up to 256, or 1024");
         if (per_cu > 64) throw Err(MFG_ERR_INVALID, "blocks per CU above 64");
         c->block = block ? block : 256;
         c->blocks_per_cu = per_cu;
@@ -1975,9 +2000,11 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
             // one untimed probe per shape loads the code objects and fills the lanes
             c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
             run_variants(c, nd.data(), seeds.data(), 0, nullptr);
-            for (int i = 0; i < nw; ++i)
+            for (int i = 0; i < nw + 1; ++i)
                 for (uint32_t b : blocks) {
-                    c->waves = wl[i]; c->block = b; c->blocks_per_cu = 0;
+                    // last: one 1024-lane block per CU, where the kernel exists
+                    if (i == nw && (b != blocks[0] || !mfg_trace_has_cu_block(feats))) continue;
+                    c->waves = i < nw ? wl[i] : 4; c->block = i < nw ? b : 1024; c->blocks_per_cu = 0;
                     mfg_step_result sr{};
                     run_variants(c, nd.data(), seeds.data(), 0, &sr);
                     double hs = sr.kernel_ms > 0 ? (double)sr.hits / (sr.kernel_ms * 1e-3) : 0.0;
@@ -1985,7 +2012,7 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
                         best = hs;
                         int grid, block, waves;
                         launch_shape(c, feats, grid, block, waves);
-                        r.waves = (uint32_t)waves; r.block = b; r.blocks_per_cu = (uint32_t)(grid / c->n_cu);
+                        r.waves = (uint32_t)waves; r.block = (uint32_t)block; r.blocks_per_cu = (uint32_t)(grid / c->n_cu);
                         r.hits_per_s = hs; r.bvh_width = (uint32_t)width; r.leaf_size = (uint32_t)leaf;
                     }
                 }

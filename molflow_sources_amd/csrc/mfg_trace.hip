@@ -1789,6 +1789,14 @@ __global__ void __launch_bounds__(256, WAVES) k_trace(TraceParams P) {
     trace_body<FEAT>(P);
 }
 
+// One block of 1024 lanes per CU at 4 waves per SIMD: the 16 waves of a CU
+// share one LDS copy of the top of the tree, which can then hold four times
+// the nodes of a 256-lane block's copy (same per-lane LDS budget).
+template <uint32_t FEAT>
+__global__ void __launch_bounds__(1024, 4) k_trace_cu(TraceParams P) {
+    trace_body<FEAT>(P);
+}
+
 // Parked particles that a launch did not claim: the queue hands out items
 // [0, claimed) and the first n_in of them are the resumed particles, so
 // [min(claimed, n_in), n_in) were never taken. That happens only when the
@@ -1881,6 +1889,13 @@ uint32_t mfg_trace_kernel_features(uint32_t model_features) {
 namespace {
 typedef void (*TraceKernel)(TraceParams);
 // [feature set][waves per SIMD - 1]; null where not instantiated
+// The 1024-lane kernels (lean and low-flux feature sets, 4 waves per SIMD).
+TraceKernel kernel_cu(uint32_t features) {
+    if (features == kLean) return k_trace_cu<kLean>;
+    if (features == kLowFlux) return k_trace_cu<kLowFlux>;
+    return nullptr;
+}
+
 TraceKernel kernel_for(uint32_t features, int waves) {
     constexpr int N = 5;   // occupancy bounds 1..5 waves per SIMD
     static TraceKernel lean[N] = {nullptr, k_trace<kLean, 2>, k_trace<kLean, 3>, k_trace<kLean, 4>, k_trace<kLean, 5>};
@@ -1913,16 +1928,25 @@ int mfg_trace_variants(uint32_t features, int* waves_out, int cap) {
     return n;
 }
 
+int mfg_trace_has_cu_block(uint32_t features) {
+    return kernel_cu(mfg_trace_kernel_features(features)) != nullptr;
+}
+
 int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds) {
     int n = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel_for(mfg_trace_kernel_features(features), waves), block, lds);
+    const uint32_t f = mfg_trace_kernel_features(features);
+    TraceKernel k = block > 256 ? kernel_cu(f) : kernel_for(f, waves);
+    if (!k) return 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, block, lds);
     return e == hipSuccess ? n : 1;
 }
 
 int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int grid, int block, void* stream) {
     size_t lds = (size_t)block * p.stack_size * sizeof(int32_t) + (size_t)p.n_hot * 8 * sizeof(double) +
                  (size_t)p.lds_nodes * sizeof(BvhNode);
-    TraceKernel k = kernel_for(mfg_trace_kernel_features(features), waves);
+    const uint32_t f = mfg_trace_kernel_features(features);
+    TraceKernel k = block > 256 ? kernel_cu(f) : kernel_for(f, waves);
+    if (!k) return -1;
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
