@@ -587,11 +587,23 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
         float tn[8];
         uint32_t m = node_hits(nd, r, tbf, tn);
-        // leaf children first (one loop for all slots, so lanes whose leaves
-        // sit in different child slots test their facets together)
         const uint32_t leaves = nd.c6.z & 0xffu;   // BvhNode::leaf_mask
         uint32_t lm = m & leaves;
         const uint32_t im = m & ~leaves;
+        // inner children first, pushed in slot order (the next node is the
+        // top of the stack), so the node's boxes are dead during the facet
+        // tests. Pushing them after the leaves, against the tmax the leaves
+        // may shrink, culled no more of them (10.58 pushes per diffuse
+        // beam-pipe ray either way).
+        // A near-to-far sort of the children leaves the node visits per ray
+        // within 2 % on these trees but costs about a hundred instructions
+        // per node, and the nearest hit does not depend on the visiting
+        // order (ties go to the lowest facet id)
+#pragma unroll
+        for (int c = 7; c >= 0; --c)
+            if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
+        // leaf children (one loop for all slots, so lanes whose leaves sit in
+        // different child slots test their facets together)
         while (lm) {
             uint32_t c = (uint32_t)__builtin_ctz(lm);
             lm &= lm - 1u;
@@ -616,15 +628,6 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
                 }
             }
         }
-        // inner children that survive the (possibly shrunk) tmax, pushed in
-        // slot order; the next node is the top of the stack. A near-to-far
-        // sort of the children leaves the node visits per ray within 2 % on
-        // these trees but costs about a hundred instructions per node, and
-        // the nearest hit does not depend on the visiting order (ties go to
-        // the lowest facet id)
-#pragma unroll
-        for (int c = 7; c >= 0; --c)
-            if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
         if (sp == 0) break;
         node = S.pop(sp);
     }
