@@ -343,7 +343,7 @@ __device__ __forceinline__ uint32_t node_count(const NodeRegs& n, uint32_t c) {
 // 2^e as a float (|e| < 127)
 __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
-// Slab test of the node's children against [0, tmax]: hit mask + entry t.
+// Slab test of the node's children against [0, tmax]: the hit mask.
 // A bound q (8 bits) on an axis sits at o + q 2^e, so its ray parameter is
 // (o + q 2^e - ray_o) / d = fma(q, 2^e/d, (o - ray_o)/d): two per-node
 // constants per axis, then one fma per bound. The quantised boxes enclose
@@ -353,7 +353,7 @@ __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) 
 // 1e-5 extent / |d| of its parameter inside the padded one (within
 // [0, tmax] as well, since tmax is at or beyond that hit), against FP32
 // errors near 1e-7 extent / |d|, so no tolerance is added to a <= b.
-__device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax, float tn[8]) {
+__device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax) {
     const float ox = __uint_as_float(n.c0.x), oy = __uint_as_float(n.c0.y), oz = __uint_as_float(n.c0.z);
     const uint32_t ew = n.c0.w;
     const float ax = exp2i((int)(int8_t)(ew & 0xffu)) * r.ix, ay = exp2i((int)(int8_t)((ew >> 8) & 0xffu)) * r.iy;
@@ -378,7 +378,6 @@ __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r,
         const float tnz = fmaf((float)((nz[h] >> sh) & 0xffu), az, bz), tfz = fmaf((float)((fz[h] >> sh) & 0xffu), az, bz);
         float a = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
         float b = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        tn[c] = a;
         if (a <= b) mask |= 1u << c;
     }
     return mask & node_valid(n);
@@ -585,14 +584,13 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
     int32_t node = root;
     for (;;) {
         const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, node);
-        float tn[8];
-        uint32_t m = node_hits(nd, r, tbf, tn);
+        uint32_t m = node_hits(nd, r, tbf);
         const uint32_t leaves = nd.c6.z & 0xffu;   // BvhNode::leaf_mask
         uint32_t lm = m & leaves;
         const uint32_t im = m & ~leaves;
         // inner children first, pushed in slot order (the next node is the
         // top of the stack), so the node's boxes are dead during the facet
-        // tests. Pushing them after the leaves, against the tmax the leaves
+        // tests; a child in the mask already enters before tmax. Pushing them after the leaves, against the tmax the leaves
         // may shrink, culled no more of them (10.58 pushes per diffuse
         // beam-pipe ray either way).
         // A near-to-far sort of the children leaves the node visits per ray
@@ -601,7 +599,7 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         // order (ties go to the lowest facet id)
 #pragma unroll
         for (int c = 7; c >= 0; --c)
-            if ((im & (1u << c)) && !(tn[c] > tbf)) S.push(sp, node_child(nd, c));
+            if (im & (1u << c)) S.push(sp, node_child(nd, c));
         // leaf children (one loop for all slots, so lanes whose leaves sit in
         // different child slots test their facets together)
         while (lm) {
@@ -1476,8 +1474,7 @@ struct Tracer {
                 double save_u = hu, save_v = hv;
                 for (;;) {
                     const NodeRegs nd = load_node(P.nodes + node);
-                    float tn[8];
-                    uint32_t msk = node_hits(nd, r, tbf, tn);
+                    uint32_t msk = node_hits(nd, r, tbf);
                     while (msk) {
                         uint32_t c = (uint32_t)__builtin_ctz(msk);
                         msk &= msk - 1u;
