@@ -371,6 +371,13 @@ def plain_leg(args, tube):
     if tube:
         # facet 1 is the outlet disk: transmission = absorbed there / desorbed
         out["config"]["transmission"] = float(t.facets["nb_abs_equiv"][0][1]) / float(t.glob[0]["nb_desorbed"])
+    if tube:
+        # 3,202 facets: nodes, leaf records, facet data and tallies (about
+        # 2 MB) stay in each XCD's 4 MB L2, so the HBM fraction is nominal
+        # here; the issue object (VALU issue and memory wait) is the roof
+        # that binds
+        out["roofline"]["note"] = ("working set L2-resident (~2 MB of nodes, leaf records, facet data and tallies); "
+                                   "the HBM fraction is nominal, the issue object is the binding roof")
     if out["cpu_baseline"]:
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     return out, info

@@ -293,7 +293,8 @@ int mfg_load_model(mfg_context* ctx, const mfg_model_view* model);
  * BuildAccelStructure (MolflowSimulation.cpp:75-88, MolflowSimGeom.cpp:124-184).
  * width = children per node (4 or 8; 0 = default 8; nodes are one 128 B
  * line with 8-bit quantised child boxes either way), leaf_size = max facets
- * per leaf (0 = default 2, the reference's setting). */
+ * per leaf (1..16; 0 = default 2, the reference's setting). MFG_ERR_LIMIT
+ * when the leaves hold 2^27 facet records or more. */
 int mfg_build_accel(mfg_context* ctx, int width, int leaf_size);
 
 /* Zero tallies and restart the particle stream for a seed. Replaces

@@ -1165,7 +1165,9 @@ void build_accel(mfg_context* c, int width, int leaf) {
     if (width != 4 && width != 8) throw Err(MFG_ERR_INVALID, "BVH width must be 4 or 8 (0 = 8)");
     c->bvh_width = width;
     if (leaf <= 0) leaf = 2;
-    if (leaf > 255) throw Err(MFG_ERR_INVALID, "leaf size above 255");
+    // a traversal stack entry holds a leaf as 27 bits of first record and
+    // 4 bits of facet count
+    if (leaf > 16) throw Err(MFG_ERR_INVALID, "leaf size above 16");
     c->leaf_size = leaf;
     size_t nv = c->models.size();
     std::vector<std::vector<BvhNode>> vnodes(nv);
@@ -1228,6 +1230,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
     uint32_t need = (uint32_t)((width - 1) * depth + width);
     if (need > MFG_MAX_STACK) throw Err(MFG_ERR_LIMIT, "BVH too deep for the LDS traversal stack (depth " + std::to_string(depth) + ")");
     c->stack_size = need;
+    if (leafs.size() >= ((size_t)1 << 27)) throw Err(MFG_ERR_LIMIT, "more than 2^27 BVH leaf records");
     hipStream_t s = c->stream;
     c->nodes.upload(nodes, s);
     // leaf-ordered copies of the test data (each facet sits in one leaf)
