@@ -182,6 +182,29 @@ def atomic_peak(device=0, n_addr=1 << 20, per_thread=64):
             "ops": "one FP64 (or uint64) atomic add without return per lane, 8 blocks of 256 per CU"}
 
 
+def gather_rates(device=0, sizes_mib=(64, 1024), per_thread=64):
+    """Rates at which this GPU reads whole 128 B lines at random indices of
+    an array of each size (libmfg_microbench.so): the leaf-record and node
+    access pattern of the trace kernel. 64 MiB is about the beam pipe's
+    working set (Infinity Cache resident), 1 GiB comes from HBM."""
+    import ctypes
+    path = os.path.join(ROOT, "molflow_sources_amd", "libmfg_microbench.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    if not hasattr(lib, "mfgb_gather_rate"):
+        return None
+    lib.mfgb_gather_rate.restype = ctypes.c_int
+    lib.mfgb_gather_rate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for mib in sizes_mib:
+        r = ctypes.c_double()
+        if lib.mfgb_gather_rate(device, mib << 20, per_thread, ctypes.byref(r)) != 0:
+            return None
+        out[f"{mib}MiB_GBps"] = r.value / 1e9
+    return out
+
+
 def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +394,14 @@ def plain_leg(args, tube):
     if tube:
         # facet 1 is the outlet disk: transmission = absorbed there / desorbed
         out["config"]["transmission"] = float(t.facets["nb_abs_equiv"][0][1]) / float(t.glob[0]["nb_desorbed"])
+    g = gather_rates()
+    if traffic and g:
+        # the measured L2-miss traffic against the rate the same access
+        # pattern (random 128 B lines) reaches at this working-set size
+        rate = traffic / launch_s / 1e9
+        out["roofline"]["traffic_GBps"] = rate
+        out["roofline"]["random_line_read_GBps"] = g
+        out["roofline"]["traffic_frac_of_random_line_rate"] = rate / g["64MiB_GBps"]
     if tube:
         # 3,202 facets: nodes, leaf records, facet data and tallies (about
         # 2 MB) stay in each XCD's 4 MB L2, so the HBM fraction is nominal

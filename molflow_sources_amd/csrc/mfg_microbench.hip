@@ -30,6 +30,25 @@ __global__ void k_atomic_u64(unsigned long long* buf, uint32_t n, uint32_t per_t
     }
 }
 
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// each thread reads per_thread whole 128 B lines at hashed line indices
+// (< n_lines) and writes one value, so no load is dead
+__global__ void k_gather(const uint4* __restrict__ a, uint32_t n_lines, uint32_t per_thread, uint32_t* out) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t h = mix32(tid * 2654435761u + 1u), s = 0;
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        const uint4* p = a + (size_t)(h % n_lines) * 8;
+        const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3], v4 = p[4], v5 = p[5], v6 = p[6], v7 = p[7];
+        s += v0.x ^ v1.y ^ v2.z ^ v3.w ^ v4.x ^ v5.y ^ v6.z ^ v7.w;
+        h = mix32(h + k);
+    }
+    out[tid] = s;
+}
+
 }  // namespace
 
 extern "C" int mfgb_atomic_rates(int device, uint32_t n_addr, uint32_t per_thread, double* f64_per_s,
@@ -68,6 +87,39 @@ extern "C" int mfgb_atomic_rates(int device, uint32_t n_addr, uint32_t per_threa
     } while (false);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    return rc;
+}
+
+extern "C" int mfgb_gather_rate(int device, uint64_t bytes, uint32_t per_thread, double* bytes_per_s) {
+    if (bytes < (1u << 20) || bytes / 128 > 0xffffffffull || per_thread == 0 || !bytes_per_s) return 1;
+    if (hipSetDevice(device) != hipSuccess) return 2;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 2;
+    const int block = 256, grid = prop.multiProcessorCount * 16;
+    const uint32_t n_lines = (uint32_t)(bytes / 128);
+    void* buf = nullptr;
+    void* out = nullptr;
+    if (hipMalloc(&buf, (size_t)n_lines * 128) != hipSuccess) return 2;
+    if (hipMalloc(&out, (size_t)grid * block * sizeof(uint32_t)) != hipSuccess) { (void)hipFree(buf); return 2; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 2;
+    float ms = 0;
+    do {
+        if (hipMemset(buf, 1, (size_t)n_lines * 128) != hipSuccess) break;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) break;
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(block), 0, 0, (const uint4*)buf, n_lines, per_thread, (uint32_t*)out);
+        if (hipDeviceSynchronize() != hipSuccess) break;
+        if (hipEventRecord(e0, 0) != hipSuccess) break;
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(block), 0, 0, (const uint4*)buf, n_lines, per_thread, (uint32_t*)out);
+        if (hipEventRecord(e1, 0) != hipSuccess) break;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+        *bytes_per_s = (double)grid * block * per_thread * 128.0 / (ms * 1e-3);
+        rc = hipGetLastError() == hipSuccess ? 0 : 2;
+    } while (false);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
     (void)hipFree(buf);
     return rc;
 }

@@ -125,8 +125,8 @@ def test_python_moment_parser_matches_oracle(oracle_lib):
 
 
 def test_microbench_library_exports_its_header():
-    """The atomic-throughput helper (measurement only) loads and exports the
-    symbol its header declares; no GPU call is made."""
+    """The measurement helper (atomic throughput, random-line read rate)
+    loads and exports the symbols its header declares; no GPU call is made."""
     import ctypes
     import re
     lib_path = os.path.join(ROOT, "molflow_sources_amd", "libmfg_microbench.so")
@@ -134,7 +134,7 @@ def test_microbench_library_exports_its_header():
         pytest.skip("libmfg_microbench.so not built")
     txt = open(os.path.join(ROOT, "include", "mfg_microbench.h")).read()
     names = re.findall(r"^int\s+(mfgb_\w+)\(", txt, flags=re.M)
-    assert names == ["mfgb_atomic_rates"]
+    assert names == ["mfgb_atomic_rates", "mfgb_gather_rate"]
     lib = ctypes.CDLL(lib_path)
     for n in names:
         assert hasattr(lib, n)
