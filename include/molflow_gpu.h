@@ -421,11 +421,12 @@ int mfg_run_variants(mfg_context* ctx, const uint64_t* n_des, const uint64_t* se
 int mfg_get_variant_layout(const mfg_context* ctx, uint32_t variant, mfg_tally_layout* layout);
 int mfg_read_variant_tallies(mfg_context* ctx, uint32_t variant, const mfg_tally_view* out);
 /* Launch shape used by runs: threads per block (a multiple of 64 up to
- * 256, or 1024) and blocks per CU (0 = defaults / autotuned). 1024 runs one
- * block per CU at 4 waves per SIMD, whose LDS copy of the top of the BVH is
- * then shared by the whole CU; it applies to models without transparency,
- * moments, volume events and the other features of the full kernel (those
- * runs take 256-thread blocks). MFG_ERR_INVALID for other sizes. */
+ * 256, 768 or 1024) and blocks per CU (0 = defaults / autotuned). 768 and
+ * 1024 run one block per CU at 3 and 4 waves per SIMD, whose LDS copy of
+ * the top of the BVH is then shared by the whole CU; they apply to models
+ * without transparency, moments, volume events and the other features of
+ * the full kernel (those runs take 256-thread blocks). MFG_ERR_INVALID for
+ * other sizes. */
 int mfg_set_launch(mfg_context* ctx, uint32_t block_threads, uint32_t blocks_per_cu);
 /* Particles kept in flight between runs (the particle state that persists
  * between SimulationMCStep calls, reference Particle.h:95-122 and
@@ -450,8 +451,8 @@ int mfg_set_kernel_variant(mfg_context* ctx, uint32_t waves_per_simd);
 
 /* Launch autotuner: times short probe runs of n_probe desorptions for every
  * BVH shape (8-wide with 2 or 1 facets per leaf, 4-wide with 2), compiled
- * occupancy bound and block size (128, 256, and 1024 where the model's
- * kernel has it) on the loaded model(s) (particles stay in
+ * occupancy bound and block size (128, 256, and 768 and 1024 where the
+ * model's kernel has them) on the loaded model(s) (particles stay in
  * flight between probes), keeps the
  * fastest, then zeroes the tallies and puts the particle streams back where
  * they were, so the probes leave no trace in later results. */

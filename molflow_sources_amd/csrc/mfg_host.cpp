@@ -1261,7 +1261,7 @@ uint32_t lds_node_count(const mfg_context* c, int block) {
     if (c->dvars.size() != 1) return 0;
     size_t n = c->node_cache;
     if (block > 256) {
-        // a 1024-lane block serves a whole CU: the LDS its stacks and hot
+        // a block of 768 or 1024 lanes serves a whole CU: the LDS its stacks and hot
         // facets leave (at least four times a 256-lane block's copy)
         const size_t used = (size_t)block * std::min<uint32_t>(c->stack_size, MFG_LDS_STACK) * sizeof(int32_t) +
                             c->hot.size() * 8 * sizeof(double);
@@ -1312,9 +1312,10 @@ void launch_shape(mfg_context* c, uint32_t feats, int& grid, int& block, int& wa
     block = (int)c->block;
     waves = c->waves ? c->waves : default_waves(feats);
     if (block > 256) {
-        // the 1024-lane kernel exists for the lean and low-flux feature
-        // sets at 4 waves per SIMD; other runs take 256-lane blocks
-        if (mfg_trace_has_cu_block(feats)) waves = 4;
+        // the one-block-per-CU kernels exist for the lean and low-flux
+        // feature sets (768 lanes: 3 waves per SIMD, 1024: 4); other runs
+        // take 256-lane blocks
+        if (mfg_trace_has_cu_block(feats, block)) waves = block / 256;
         else block = 256;
     }
     auto lds_bytes = [&](int b) {
@@ -1944,8 +1945,8 @@ int mfg_get_trace(mfg_context* c, int32_t* facet, int32_t* kind, size_t n) {
 
 int mfg_set_launch(mfg_context* c, uint32_t block, uint32_t per_cu) {
     return guarded(c, [&] {
-        if (block != 0 && block != 1024 && (block % 64 != 0 || block > 256))
-            throw Err(MFG_ERR_INVALID, "block must be a multiple of 64 up to 256, or 1024");
+        if (block != 0 && block != 768 && block != 1024 && (block % 64 != 0 || block > 256))
+            throw Err(MFG_ERR_INVALID, "block must be a multiple of 64 up to 256, 768 or 1024");
         if (per_cu > 64) throw Err(MFG_ERR_INVALID, "blocks per CU above 64");
         c->block = block ? block : 256;
         c->blocks_per_cu = per_cu;
@@ -2003,11 +2004,12 @@ int mfg_autotune(mfg_context* c, uint64_t n_probe, mfg_tune_result* out) {
             // one untimed probe per shape loads the code objects and fills the lanes
             c->block = 256; c->blocks_per_cu = 0; c->waves = wl[0];
             run_variants(c, nd.data(), seeds.data(), 0, nullptr);
-            for (int i = 0; i < nw + 1; ++i)
+            for (int i = 0; i < nw + 2; ++i)
                 for (uint32_t b : blocks) {
-                    // last: one 1024-lane block per CU, where the kernel exists
-                    if (i == nw && (b != blocks[0] || !mfg_trace_has_cu_block(feats))) continue;
-                    c->waves = i < nw ? wl[i] : 4; c->block = i < nw ? b : 1024; c->blocks_per_cu = 0;
+                    // last: one block per CU of 768 and of 1024 lanes, where the kernel exists
+                    const uint32_t cu_block = i == nw ? 768u : 1024u;
+                    if (i >= nw && (b != blocks[0] || !mfg_trace_has_cu_block(feats, (int)cu_block))) continue;
+                    c->waves = i < nw ? wl[i] : (int)cu_block / 256; c->block = i < nw ? b : cu_block; c->blocks_per_cu = 0;
                     mfg_step_result sr{};
                     run_variants(c, nd.data(), seeds.data(), 0, &sr);
                     double hs = sr.kernel_ms > 0 ? (double)sr.hits / (sr.kernel_ms * 1e-3) : 0.0;

@@ -357,6 +357,6 @@ int mfg_launch_backward(const double* jac, const double* up, double* grad, uint3
 uint32_t mfg_trace_kernel_features(uint32_t model_features);
 int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds_bytes);
 int mfg_trace_variants(uint32_t features, int* waves_out, int cap);
-// 1 when the feature set has the 1024-lane (one block per CU, 4 waves per
-// SIMD) kernel
-int mfg_trace_has_cu_block(uint32_t features);
+// 1 when the feature set has a one-block-per-CU kernel of `block` lanes
+// (768: 3 waves per SIMD, 1024: 4)
+int mfg_trace_has_cu_block(uint32_t features, int block);

@@ -1813,11 +1813,12 @@ __global__ void __launch_bounds__(256, WAVES) k_trace(TraceParams P) {
     trace_body<FEAT>(P);
 }
 
-// One block of 1024 lanes per CU at 4 waves per SIMD: the 16 waves of a CU
-// share one LDS copy of the top of the tree, which can then hold four times
-// the nodes of a 256-lane block's copy (same per-lane LDS budget).
-template <uint32_t FEAT>
-__global__ void __launch_bounds__(1024, 4) k_trace_cu(TraceParams P) {
+// One block per CU of 256 lanes per wave slot (768 lanes at 3 waves per
+// SIMD, 1024 at 4): all waves of a CU share one LDS copy of the top of the
+// tree, which can then hold several times the nodes of a 256-lane block's
+// copy (same per-lane LDS budget).
+template <uint32_t FEAT, int WAVES>
+__global__ void __launch_bounds__(256 * WAVES, WAVES) k_trace_cu(TraceParams P) {
     trace_body<FEAT>(P);
 }
 
@@ -1913,10 +1914,16 @@ uint32_t mfg_trace_kernel_features(uint32_t model_features) {
 namespace {
 typedef void (*TraceKernel)(TraceParams);
 // [feature set][waves per SIMD - 1]; null where not instantiated
-// The 1024-lane kernels (lean and low-flux feature sets, 4 waves per SIMD).
-TraceKernel kernel_cu(uint32_t features) {
-    if (features == kLean) return k_trace_cu<kLean>;
-    if (features == kLowFlux) return k_trace_cu<kLowFlux>;
+// The one-block-per-CU kernels (lean and low-flux feature sets, 3 or 4
+// waves per SIMD: 768 or 1024 lanes).
+TraceKernel kernel_cu(uint32_t features, int waves) {
+    if (waves == 3) {
+        if (features == kLean) return k_trace_cu<kLean, 3>;
+        if (features == kLowFlux) return k_trace_cu<kLowFlux, 3>;
+    } else if (waves == 4) {
+        if (features == kLean) return k_trace_cu<kLean, 4>;
+        if (features == kLowFlux) return k_trace_cu<kLowFlux, 4>;
+    }
     return nullptr;
 }
 
@@ -1952,14 +1959,14 @@ int mfg_trace_variants(uint32_t features, int* waves_out, int cap) {
     return n;
 }
 
-int mfg_trace_has_cu_block(uint32_t features) {
-    return kernel_cu(mfg_trace_kernel_features(features)) != nullptr;
+int mfg_trace_has_cu_block(uint32_t features, int block) {
+    return block % 256 == 0 && kernel_cu(mfg_trace_kernel_features(features), block / 256) != nullptr;
 }
 
 int mfg_trace_occupancy(uint32_t features, int waves, int block, size_t lds) {
     int n = 0;
     const uint32_t f = mfg_trace_kernel_features(features);
-    TraceKernel k = block > 256 ? kernel_cu(f) : kernel_for(f, waves);
+    TraceKernel k = block > 256 ? kernel_cu(f, block / 256) : kernel_for(f, waves);
     if (!k) return 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, block, lds);
     return e == hipSuccess ? n : 1;
@@ -1969,7 +1976,7 @@ int mfg_launch_trace(const TraceParams& p, uint32_t features, int waves, int gri
     size_t lds = (size_t)block * p.stack_size * sizeof(int32_t) + (size_t)p.n_hot * 8 * sizeof(double) +
                  (size_t)p.lds_nodes * sizeof(BvhNode);
     const uint32_t f = mfg_trace_kernel_features(features);
-    TraceKernel k = block > 256 ? kernel_cu(f) : kernel_for(f, waves);
+    TraceKernel k = block > 256 ? kernel_cu(f, block / 256) : kernel_for(f, waves);
     if (!k) return -1;
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -119,8 +119,9 @@ class GpuSimulation:
         self._check(self._lib.mfg_set_time_limit(self.h, float(seconds)), "mfg_set_time_limit")
 
     def set_launch(self, block: int = 256, blocks_per_cu: int = 0) -> None:
-        """Threads per block (a multiple of 64 up to 256, or 1024 for one
-        block per CU) and blocks per CU (0 = fill each CU)."""
+        """Threads per block (a multiple of 64 up to 256, or 768 / 1024 for
+        one block per CU at 3 / 4 waves per SIMD) and blocks per CU (0 =
+        fill each CU)."""
         self._check(self._lib.mfg_set_launch(self.h, block, blocks_per_cu), "mfg_set_launch")
 
     def set_carry(self, on: bool = True) -> None:

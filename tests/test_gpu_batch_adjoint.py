@@ -58,7 +58,7 @@ def test_launch_shape_does_not_change_results(sim):
     sim.load(m)
     sim.rebuild_accel()
     res = []
-    for block, per_cu in ((256, 0), (64, 3), (128, 1), (1024, 0)):
+    for block, per_cu in ((256, 0), (64, 3), (128, 1), (768, 0), (1024, 0)):
         sim.set_launch(block, per_cu)
         sim.reset(seed=9)
         sim.run(4000, max_events=MAXEV)

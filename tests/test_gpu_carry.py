@@ -88,7 +88,7 @@ def test_carry_across_launch_shapes_and_variants(sim):
     sim.reset(seed=5)
     sim.set_carry(True)
     # shrinking and growing the grid keeps the parked particles
-    for (block, per_cu, waves) in ((64, 1, 0), (1024, 0, 0), (128, 0, 3), (256, 0, 1)):
+    for (block, per_cu, waves) in ((64, 1, 0), (1024, 0, 0), (768, 0, 0), (256, 0, 1)):
         sim.set_launch(block, per_cu)
         sim.set_kernel_variant(waves)
         sim.run(50_000, max_events=MAXEV)
