@@ -132,14 +132,46 @@ KERNEL_SOURCES = ("molflow_sources_amd/csrc/mfg_trace.hip", "molflow_sources_amd
                   "include/molflow_gpu.h")
 
 
+def code_only(text):
+    """C/C++ source without comments, trailing blanks and empty lines (string
+    and character literals kept as written)."""
+    out, i, n, quote = [], 0, len(text), None
+    while i < n:
+        c = text[i]
+        if quote:
+            out.append(c)
+            if c == "\\" and i + 1 < n:
+                out.append(text[i + 1])
+                i += 2
+                continue
+            if c == quote:
+                quote = None
+            i += 1
+        elif c in "\"'":
+            quote = c
+            out.append(c)
+            i += 1
+        elif text.startswith("//", i):
+            j = text.find("\n", i)
+            i = n if j < 0 else j
+        elif text.startswith("/*", i):
+            j = text.find("*/", i + 2)
+            i = n if j < 0 else j + 2
+        else:
+            out.append(c)
+            i += 1
+    return "\n".join(l.rstrip() for l in "".join(out).splitlines() if l.strip())
+
+
 def kernel_source_hash():
-    """sha256 of the kernel sources (the trace kernel and the layouts it
-    reads): a profile summary applies to this build only if it records it."""
+    """sha256 of the code of the kernel sources (the trace kernel and the
+    layouts it reads; comments and blank lines left out): a profile summary
+    applies to this build only if it records it."""
     import hashlib
     h = hashlib.sha256()
     for f in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, f), "rb") as fh:
-            h.update(fh.read())
+        with open(os.path.join(ROOT, f), encoding="utf-8") as fh:
+            h.update(code_only(fh.read()).encode())
     return h.hexdigest()
 
 

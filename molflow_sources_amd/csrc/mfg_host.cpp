@@ -5,8 +5,9 @@
 // texture cell increments, Initializer.cpp:433-456; Maxwell CDF,
 // CDFGeneration.cpp:55-83; integrated desorption, IDGeneration.cpp:45-206;
 // moments, TimeMoments.cpp:58-83), builds the per-structure BVH
-// (BuildAccelStructure, MolflowSimGeom.cpp:124-184) as 4-wide nodes, and
-// moves everything to HBM once. Runs launch the persistent trace kernel.
+// (BuildAccelStructure, MolflowSimGeom.cpp:124-184) as 8-wide (or 4-wide)
+// quantised nodes, and moves everything to HBM once. Runs launch the
+// persistent trace kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -298,10 +299,9 @@ void build_b2(std::vector<B2>& nodes, std::vector<int>& prims, const std::vector
 inline float round_down(double x) { float f = (float)x; if ((double)f > x) f = std::nextafter(f, -INFINITY); return f; }
 inline float round_up(double x) { float f = (float)x; if ((double)f < x) f = std::nextafter(f, INFINITY); return f; }
 
-// Collapse a binary tree into 4-wide nodes (open the largest inner child
-// until 4 children), boxes padded by `pad` and rounded outwards to FP32.
-// Quantised child bounds of one axis: the FP32 values the kernel computes,
-// o + q * 2^e (one rounding), must enclose [lo, hi].
+// Quantised child bounds of one axis (boxes padded by `pad` and rounded
+// outwards): the FP32 values the kernel computes, o + q * 2^e (one
+// rounding), must enclose [lo, hi].
 struct AxisQuant {
     float o;
     int e;

@@ -2,14 +2,18 @@
 //
 // k_trace: one persistent launch traces a whole batch of desorptions. Each
 // lane owns one particle in registers; dead lanes refill from a global work
-// queue with one atomic per wave (ballot + mbcnt). Per event a lane walks a
-// 4-wide BVH (FP32 boxes, traversal stack in LDS laid out [depth][lane] so a
-// wave's pushes hit distinct banks), tests facets in FP64, and applies the
-// event of reference src/Simulation/Particle.cpp:207-396 (desorption, bounce,
-// absorption, transparent pass, teleport, link, leak, volume events).
-// Tallies go straight to HBM with u64 / FP64 atomics; the desorption tallies
-// of lanes that share a source facet are summed across the wave first, and
-// the global counters are kept per lane and flushed once per wave at exit.
+// queue with one atomic per wave (ballot + mbcnt), first from the particles
+// the previous launch parked. Per event a lane walks an 8-wide BVH of 128 B
+// nodes with 8-bit quantised FP32 boxes (the top of the tree copied to LDS
+// per block, traversal stack in LDS laid out [entry][lane]); leaf children
+// go on the stack and the wave tests them together once enough lanes hold
+// one; facets are tested in FP64. It then applies the event of reference
+// src/Simulation/Particle.cpp:207-396 (desorption, bounce, absorption,
+// transparent pass, teleport, link, leak, volume events). Tallies are u64 /
+// FP64 atomics: lanes of a wave that update the same facet counter (or
+// desorb from the same facet) add through one set of atomics, the counters
+// of a few hot facets are summed per block in LDS, and the global counters
+// are kept per lane and flushed once per wave at exit.
 //
 // Random numbers: Philox4x32-10 keyed by the variant seed, counter
 // (particle index, block, 0); a lane draws them in the same order as the

@@ -220,3 +220,15 @@ def test_oracle_adjoint_refuses_sticking_outside_the_open_interval(oracle_lib):
                              j0.ctypes.data_as(C2.POINTER(C2.c_double)), jac.ctypes.data_as(C2.POINTER(C2.c_double)),
                              1000)
     assert rc == 0 and np.all(np.isfinite(jac))
+
+
+def test_kernel_hash_ignores_comments_only():
+    """bench.py keys the profile summaries to the code of the kernel
+    sources: comments and blank lines do not change the key, code does, and
+    comment markers inside literals are code."""
+    import bench
+    a = 'int a = 1; // note\n/* block\n comment */ char q = \'/\';\n\nconst char* t = "x // y";\n'
+    b = 'int a = 1;\n char q = \'/\';\nconst char* t = "x // y";   \n'
+    assert bench.code_only(a) == bench.code_only(b)
+    assert bench.code_only(a) != bench.code_only(a.replace("a = 1", "a = 2"))
+    assert "x // y" in bench.code_only(a)
