@@ -76,6 +76,23 @@ def test_first_hit_bit_exact_4_wide(gpu):
     assert np.all(np.abs(tg[hit] - tc[hit]) <= 1e-5 * tc[hit])
 
 
+def test_first_hit_bit_exact_sixteen_facet_leaves(gpu):
+    """The largest leaf a traversal stack entry holds (16 facets, a 4-bit
+    count of 15) on the beam pipe: the same first hits as brute force; a
+    leaf size of 17 is refused."""
+    from molflow_sources_amd.simulation import MolflowGpuError
+    m = geometry.beam_pipe_model(n_az=48, n_ax=120, port_rings=6, n_textures=0, n_anglemaps=0)
+    o, d = random_rays(30_000, [-3.9, -3.9, 1.0], [3.9, 3.9, 199.0], seed=37)
+    gpu.load(m)
+    gpu.rebuild_accel(8, 16)
+    fg, tg, _, _ = gpu.intersect(o, d)
+    fc, tc, _, _ = CpuModel(m).intersect(o, d, brute=True)
+    assert np.array_equal(fg, fc), f"{int((fg != fc).sum())} first-hit facets differ"
+    with pytest.raises(MolflowGpuError):
+        gpu.rebuild_accel(8, 17)
+    gpu.rebuild_accel()
+
+
 @pytest.mark.parametrize("width,leaf", [(8, 1), (4, 1)])
 def test_first_hit_bit_exact_one_facet_leaves(gpu, width, leaf):
     """Leaves of one facet (the shape the autotuner picks for the tube) on
