@@ -1216,6 +1216,10 @@ void build_accel(mfg_context* c, int width, int leaf) {
         for (BvhNode nd : vnodes[vi]) {
             for (int k = 0; k < 8; ++k) {
                 if (!(nd.valid & (1u << k))) continue;
+                // a leaf's facet count travels in 4 bits of a stack entry
+                if ((nd.leaf_mask >> k) & 1u) {
+                    if (nd.count[k] < 1 || nd.count[k] > 16) throw Err(MFG_ERR_INVALID, "BVH leaf outside 1..16 facets");
+                }
                 if (nd.count[k] > 0) nd.child[k] = ~((~nd.child[k]) + lb);
                 else nd.child[k] += nb;
             }
