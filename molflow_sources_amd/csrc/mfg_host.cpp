@@ -331,12 +331,25 @@ struct AxisQuant {
     }
 };
 
+// A wide node while the tree is built: explicit child indices, which
+// pack_node turns into the device node's implicit indexing.
+struct WideNode {
+    float o[3];
+    int8_t e[3];
+    uint8_t valid, leaf_mask;
+    uint8_t qlo[3][8], qhi[3][8];   // steps 0..255 (an empty slot: lo 255, hi 0)
+    int32_t child[8];               // inner: node index; leaf: first leaf record
+    uint8_t count[8];               // facets of a leaf child (0 for inner/empty)
+};
+
 // Collapses the binary tree into nodes of up to `width` children (4 or 8):
-// the child with the largest surface is opened until the node is full.
+// the child with the largest surface is opened until the node is full. A
+// node's leaf records are appended before its inner children are built, so
+// they are consecutive in slot order.
 struct Collapse {
     const std::vector<B2>& b2;
     const std::vector<int>& prims;
-    std::vector<BvhNode>& out;
+    std::vector<WideNode>& out;
     std::vector<int32_t>& leaf_ids;
     int32_t node_base, leaf_base;
     double pad;
@@ -357,19 +370,17 @@ struct Collapse {
             ch[nc++] = b2[c].right;
         }
         int idx = (int)out.size();
-        out.push_back(BvhNode());
-        BvhNode nd;
+        out.push_back(WideNode());
+        WideNode nd;
         std::memset(&nd, 0, sizeof(nd));
         Box all = empty_box();
         for (int c = 0; c < nc; ++c) grow(all, b2[ch[c]].b);
         AxisQuant q[3];
         for (int k = 0; k < 3; ++k) q[k] = AxisQuant::make(all.lo[k] - pad, all.hi[k] + pad);
-        nd.ox = q[0].o; nd.oy = q[1].o; nd.oz = q[2].o;
-        nd.ex = (int8_t)q[0].e; nd.ey = (int8_t)q[1].e; nd.ez = (int8_t)q[2].e;
+        for (int k = 0; k < 3; ++k) { nd.o[k] = q[k].o; nd.e[k] = (int8_t)q[k].e; }
         for (int c = 0; c < 8; ++c) {
             if (c >= nc) {   // empty slot: an inverted box, never hit
                 for (int k = 0; k < 3; ++k) { nd.qlo[k][c] = 255; nd.qhi[k][c] = 0; }
-                nd.child[c] = 0; nd.count[c] = 0;
                 continue;
             }
             const B2& cn = b2[ch[c]];
@@ -378,22 +389,56 @@ struct Collapse {
                 nd.qlo[k][c] = q[k].qlo(cn.b.lo[k] - pad);
                 nd.qhi[k][c] = q[k].qhi(cn.b.hi[k] + pad);
             }
-            if (cn.left < 0) {
-                if (cn.count > 255) throw Err(MFG_ERR_LIMIT, "BVH leaf with more than 255 facets");
-                int32_t first = (int32_t)leaf_ids.size() + leaf_base;
+            if (cn.left < 0) {   // leaf: its records now, in slot order
+                if (cn.count < 1 || cn.count > 16) throw Err(MFG_ERR_INVALID, "BVH leaf outside 1..16 facets");
+                nd.child[c] = (int32_t)leaf_ids.size() + leaf_base;
                 for (int i = 0; i < cn.count; ++i) leaf_ids.push_back(prims[cn.first + i]);
-                nd.child[c] = ~first;
                 nd.count[c] = (uint8_t)cn.count;
                 nd.leaf_mask |= (uint8_t)(1u << c);
-            } else {
-                nd.child[c] = build(ch[c], depth + 1) + node_base;
-                nd.count[c] = 0;
             }
         }
+        for (int c = 0; c < nc; ++c)
+            if (b2[ch[c]].left >= 0) nd.child[c] = build(ch[c], depth + 1) + node_base;
         out[idx] = nd;
         return idx;
     }
 };
+
+// FP16 bit pattern of an integer 0..255 (exact)
+inline uint16_t half_of_step(uint8_t q) {
+    if (q == 0) return 0;
+    int e = 31 - __builtin_clz((unsigned)q);          // 2^e <= q < 2^(e+1)
+    return (uint16_t)(((e + 15) << 10) | (((unsigned)q << (10 - e)) & 0x3ffu));
+}
+
+// The device node of a wide node: inner children must be consecutive nodes
+// and leaf records consecutive in slot order (breadth-first numbering and
+// Collapse's record order give both; checked here).
+BvhNode pack_node(const WideNode& w) {
+    BvhNode p;
+    std::memset(&p, 0, sizeof(p));
+    p.ox = w.o[0]; p.oy = w.o[1]; p.oz = w.o[2];
+    p.ex = w.e[0]; p.ey = w.e[1]; p.ez = w.e[2];
+    p.leaf_mask = w.leaf_mask;
+    p.inner_mask = (uint8_t)(w.valid & ~w.leaf_mask);
+    int inner = 0, leafs = 0;
+    for (int c = 0; c < 8; ++c) {
+        for (int k = 0; k < 3; ++k) { p.qlo[k][c] = half_of_step(w.qlo[k][c]); p.qhi[k][c] = half_of_step(w.qhi[k][c]); }
+        if (!(w.valid & (1u << c))) continue;
+        if (w.leaf_mask & (1u << c)) {
+            if (leafs == 0) p.leaf_base = w.child[c];
+            if (w.child[c] != p.leaf_base + leafs) throw Err(MFG_ERR_INVALID, "BVH leaf records of a node not consecutive");
+            if (w.count[c] < 1 || w.count[c] > 16) throw Err(MFG_ERR_INVALID, "BVH leaf outside 1..16 facets");
+            p.counts |= (uint32_t)(w.count[c] - 1) << (4 * c);
+            leafs += w.count[c];
+        } else {
+            if (inner == 0) p.node_base = w.child[c];
+            if (w.child[c] != p.node_base + inner) throw Err(MFG_ERR_INVALID, "BVH inner children of a node not consecutive");
+            ++inner;
+        }
+    }
+    return p;
+}
 
 // ------------------------------------------------------------------ host model
 struct Param { std::vector<XY> pts; bool logx, logy; };
@@ -1132,26 +1177,26 @@ void zero_tallies(mfg_context* c, bool keep_particles = false) {
 
 // Renumber a variant's nodes breadth-first from its roots, so the top of
 // every tree is one prefix of the array (the part a block copies to LDS).
-void breadth_first(std::vector<BvhNode>& nodes, std::vector<int32_t>& roots) {
+void breadth_first(std::vector<WideNode>& nodes, std::vector<int32_t>& roots) {
     std::vector<int32_t> order, idx(nodes.size(), -1);
     order.reserve(nodes.size());
     for (int32_t r : roots)
         if (r >= 0) { idx[(size_t)r] = (int32_t)order.size(); order.push_back(r); }
     for (size_t h = 0; h < order.size(); ++h) {
-        const BvhNode& nd = nodes[(size_t)order[h]];
+        const WideNode& nd = nodes[(size_t)order[h]];
         for (int k = 0; k < 8; ++k) {
-            if (!(nd.valid & (1u << k)) || nd.count[k] > 0) continue;
+            if (!(nd.valid & (1u << k)) || (nd.leaf_mask & (1u << k))) continue;
             int32_t ch = nd.child[k];
             idx[(size_t)ch] = (int32_t)order.size();
             order.push_back(ch);
         }
     }
     if (order.size() != nodes.size()) throw Err(MFG_ERR_INVALID, "BVH nodes unreachable from the roots");
-    std::vector<BvhNode> out(nodes.size());
+    std::vector<WideNode> out(nodes.size());
     for (size_t i = 0; i < order.size(); ++i) {
-        BvhNode nd = nodes[(size_t)order[i]];
+        WideNode nd = nodes[(size_t)order[i]];
         for (int k = 0; k < 8; ++k)
-            if ((nd.valid & (1u << k)) && nd.count[k] == 0) nd.child[k] = idx[(size_t)nd.child[k]];
+            if ((nd.valid & (1u << k)) && !(nd.leaf_mask & (1u << k))) nd.child[k] = idx[(size_t)nd.child[k]];
         out[i] = nd;
     }
     nodes.swap(out);
@@ -1170,7 +1215,7 @@ void build_accel(mfg_context* c, int width, int leaf) {
     if (leaf > 16) throw Err(MFG_ERR_INVALID, "leaf size above 16");
     c->leaf_size = leaf;
     size_t nv = c->models.size();
-    std::vector<std::vector<BvhNode>> vnodes(nv);
+    std::vector<std::vector<WideNode>> vnodes(nv);
     std::vector<std::vector<int32_t>> vleaf(nv), vroots(nv);
     std::vector<int> vdepth(nv, 0);
     std::atomic<size_t> next{0};
@@ -1213,17 +1258,12 @@ void build_accel(mfg_context* c, int width, int leaf) {
     int depth = 0;
     for (size_t vi = 0; vi < nv; ++vi) {
         int32_t nb = (int32_t)nodes.size(), lb = (int32_t)leafs.size();
-        for (BvhNode nd : vnodes[vi]) {
+        for (WideNode nd : vnodes[vi]) {
             for (int k = 0; k < 8; ++k) {
                 if (!(nd.valid & (1u << k))) continue;
-                // a leaf's facet count travels in 4 bits of a stack entry
-                if ((nd.leaf_mask >> k) & 1u) {
-                    if (nd.count[k] < 1 || nd.count[k] > 16) throw Err(MFG_ERR_INVALID, "BVH leaf outside 1..16 facets");
-                }
-                if (nd.count[k] > 0) nd.child[k] = ~((~nd.child[k]) + lb);
-                else nd.child[k] += nb;
+                nd.child[k] += (nd.leaf_mask & (1u << k)) ? lb : nb;
             }
-            nodes.push_back(nd);
+            nodes.push_back(pack_node(nd));
         }
         leafs.insert(leafs.end(), vleaf[vi].begin(), vleaf[vi].end());
         c->dvars[vi].root_base = (uint32_t)roots.size();

@@ -119,19 +119,26 @@ struct alignas(16) DPhys {
 };
 
 // One BVH node of up to 8 children in one 128 B line. Child boxes are
-// quantised to 8 bits per plane relative to the node's origin with a
+// quantised to 8-bit steps per plane relative to the node's origin with a
 // power-of-two step per axis: lo = o + qlo * 2^e, hi = o + qhi * 2^e in FP32
 // (the builder checks that these FP32 values enclose the padded FP64 box).
+// The steps are stored as the FP16 values of those integers (exact), which
+// the kernel converts and scales in one v_fma_mix_f32. Children are found
+// without per-child words: a node's inner children are consecutive nodes
+// from node_base in slot order, and the leaf records of its leaf children
+// are consecutive from leaf_base in slot order, each leaf holding 1 + its
+// 4-bit count facets.
 struct alignas(128) BvhNode {
     float ox, oy, oz;           // quantisation origin
     int8_t ex, ey, ez;          // per-axis step exponents
-    uint8_t valid;              // bit c set when child slot c is used
-    int32_t child[8];           // >= 0: node index; < 0: leaf, first leaf record = ~child
-    uint8_t count[8];           // facets in a leaf child (0 for inner/empty)
-    uint8_t qlo[3][8];          // [axis][child]
-    uint8_t qhi[3][8];
     uint8_t leaf_mask;          // bit c set when child slot c is a leaf
-    uint8_t pad[23];
+    int32_t node_base;          // first inner child (0 without inner children)
+    int32_t leaf_base;          // first leaf record of the first leaf child (0 without leaves)
+    uint32_t counts;            // bits 4c..4c+3: facets - 1 of a leaf child in slot c
+    uint8_t inner_mask;         // bit c set when child slot c is an inner node
+    uint8_t pad[3];
+    uint16_t qlo[3][8];         // [axis][child], FP16 bit patterns of 0..255
+    uint16_t qhi[3][8];
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 bytes");
 

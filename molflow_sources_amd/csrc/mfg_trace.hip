@@ -304,18 +304,19 @@ __device__ __forceinline__ Ray32 make_ray32(d3 P, d3 D) {
     return {(float)P.x, (float)P.y, (float)P.z, safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
 }
 
-// A node's 128 B (112 used) as seven 16-byte loads issued together (one
-// round trip). Words: c0 = origin x,y,z + (ex, ey, ez, valid) bytes;
-// c1, c2 = child words 0-3, 4-7; c3 = leaf counts (x,y) + qlo x (z,w);
-// c4 = qlo y, qlo z; c5 = qhi x, qhi y; c6 = qhi z (x,y), leaf mask (z byte 0).
+// A node's 128 B as eight 16-byte loads issued together (one round trip).
+// Words: c0 = origin x,y,z + (ex, ey, ez, leaf mask) bytes; c1 = node_base,
+// leaf_base, leaf counts, inner mask (byte 0); c2..c4 = low planes of x, y,
+// z and c5..c7 = high planes, two FP16 steps per word (slot 2i low half,
+// 2i + 1 high half).
 struct NodeRegs {
-    uint4 c0, c1, c2, c3, c4, c5, c6;
+    uint4 c0, c1, c2, c3, c4, c5, c6, c7;
 };
 
 __device__ __forceinline__ NodeRegs load_node(const BvhNode* __restrict__ nd) {
     const uint4* q = reinterpret_cast<const uint4*>(nd);
     NodeRegs n;
-    n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6];
+    n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6]; n.c7 = q[7];
     return n;
 }
 
@@ -325,38 +326,52 @@ __device__ __forceinline__ NodeRegs load_node_at(const BvhNode* __restrict__ nod
     if ((uint32_t)node < n_lds) {
         const uint4* q = lnodes + (size_t)node * 8;
         NodeRegs n;
-        n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6];
+        n.c0 = q[0]; n.c1 = q[1]; n.c2 = q[2]; n.c3 = q[3]; n.c4 = q[4]; n.c5 = q[5]; n.c6 = q[6]; n.c7 = q[7];
         return n;
     }
     return load_node(nodes + node);
 }
 
-__device__ __forceinline__ uint32_t node_valid(const NodeRegs& n) { return n.c0.w >> 24; }
-// Child word of slot c (0-7) by a select tree on c's bits: 7 selects for a
-// lane-varying c (a chain of comparisons compiles to branches instead)
-__device__ __forceinline__ int32_t node_child(const NodeRegs& n, uint32_t c) {
-    const bool b2 = (c & 4u) != 0, b1 = (c & 2u) != 0, b0 = (c & 1u) != 0;
-    const uint32_t x = b2 ? n.c2.x : n.c1.x, y = b2 ? n.c2.y : n.c1.y;
-    const uint32_t z = b2 ? n.c2.z : n.c1.z, w = b2 ? n.c2.w : n.c1.w;
-    const uint32_t lo = b0 ? y : x, hi = b0 ? w : z;
-    return (int32_t)(b1 ? hi : lo);
+__device__ __forceinline__ uint32_t node_leaves(const NodeRegs& n) { return n.c0.w >> 24; }
+__device__ __forceinline__ uint32_t node_inner(const NodeRegs& n) { return n.c1.w & 0xffu; }
+// inner child of slot c: the node_base-th node plus the inner slots before c
+__device__ __forceinline__ int32_t node_inner_child(const NodeRegs& n, uint32_t c) {
+    return (int32_t)n.c1.x + (int32_t)__popc(node_inner(n) & ((1u << c) - 1u));
 }
-__device__ __forceinline__ uint32_t node_count(const NodeRegs& n, uint32_t c) {
-    return ((c < 4 ? n.c3.x : n.c3.y) >> (8u * (c & 3u))) & 0xffu;
+// leaf child of slot c: (first record, facets); its records follow those of
+// the leaf slots before c, each holding 1 + its 4-bit count facets
+__device__ __forceinline__ uint32_t nibble_sum(uint32_t v) {
+    const uint32_t t = (v & 0x0f0f0f0fu) + ((v >> 4) & 0x0f0f0f0fu);
+    return (t * 0x01010101u) >> 24;
+}
+__device__ __forceinline__ void node_leaf_child(const NodeRegs& n, uint32_t c, uint32_t& first, uint32_t& cnt) {
+    const uint32_t below = (1u << c) - 1u;
+    const uint32_t cw = n.c1.z;
+    first = n.c1.y + (uint32_t)__popc(node_leaves(n) & below) + nibble_sum(cw & ((1u << (4u * c)) - 1u));
+    cnt = ((cw >> (4u * c)) & 15u) + 1u;
 }
 // 2^e as a float (|e| < 127)
 __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+// the FP16 step of slot 2i (lo) or 2i + 1 (hi) of a plane word, as a float
+// (converted inside the v_fma_mix_f32 that uses it)
+__device__ __forceinline__ float step_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)); }
+__device__ __forceinline__ float step_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+__device__ __forceinline__ uint32_t word_of(const uint4& v, uint32_t i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
 
 // Slab test of the node's children against [0, tmax]: the hit mask.
-// A bound q (8 bits) on an axis sits at o + q 2^e, so its ray parameter is
-// (o + q 2^e - ray_o) / d = fma(q, 2^e/d, (o - ray_o)/d): two per-node
-// constants per axis, then one fma per bound. The quantised boxes enclose
-// the FP64 boxes padded by 1e-5 of the model extent, far above the FP32
-// rounding of these steps, so the test never drops a box that holds an
-// FP64 hit: a ray through a point of the unpadded box spends at least
-// 1e-5 extent / |d| of its parameter inside the padded one (within
-// [0, tmax] as well, since tmax is at or beyond that hit), against FP32
-// errors near 1e-7 extent / |d|, so no tolerance is added to a <= b.
+// A bound q (an integer step) on an axis sits at o + q 2^e, so its ray
+// parameter is (o + q 2^e - ray_o) / d = fma(q, 2^e/d, (o - ray_o)/d): two
+// per-node constants per axis, then one fma per bound (q held as an FP16
+// value, so conversion and fma are one v_fma_mix_f32 with the same single
+// rounding). The quantised boxes enclose the FP64 boxes padded by 1e-5 of
+// the model extent, far above the FP32 rounding of these steps, so the
+// test never drops a box that holds an FP64 hit: a ray through a point of
+// the unpadded box spends at least 1e-5 extent / |d| of its parameter
+// inside the padded one (within [0, tmax] as well, since tmax is at or
+// beyond that hit), against FP32 errors near 1e-7 extent / |d|, so no
+// tolerance is added to a <= b.
 __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r, float tmax) {
     const float ox = __uint_as_float(n.c0.x), oy = __uint_as_float(n.c0.y), oz = __uint_as_float(n.c0.z);
     const uint32_t ew = n.c0.w;
@@ -367,24 +382,24 @@ __device__ __forceinline__ uint32_t node_hits(const NodeRegs& n, const Ray32& r,
     // low bound is then the entry plane and the high one the exit plane, and
     // the reverse otherwise (an empty slot, low 255 > high 0, still misses)
     const bool px = r.ix >= 0.0f, py = r.iy >= 0.0f, pz = r.iz >= 0.0f;
-    const uint32_t nx[2] = {px ? n.c3.z : n.c5.x, px ? n.c3.w : n.c5.y};
-    const uint32_t fx[2] = {px ? n.c5.x : n.c3.z, px ? n.c5.y : n.c3.w};
-    const uint32_t ny[2] = {py ? n.c4.x : n.c5.z, py ? n.c4.y : n.c5.w};
-    const uint32_t fy[2] = {py ? n.c5.z : n.c4.x, py ? n.c5.w : n.c4.y};
-    const uint32_t nz[2] = {pz ? n.c4.z : n.c6.x, pz ? n.c4.w : n.c6.y};
-    const uint32_t fz[2] = {pz ? n.c6.x : n.c4.z, pz ? n.c6.y : n.c4.w};
+    const uint4 nx = px ? n.c2 : n.c5, fx = px ? n.c5 : n.c2;
+    const uint4 ny = py ? n.c3 : n.c6, fy = py ? n.c6 : n.c3;
+    const uint4 nz = pz ? n.c4 : n.c7, fz = pz ? n.c7 : n.c4;
     uint32_t mask = 0;
 #pragma unroll
     for (uint32_t c = 0; c < 8; ++c) {
-        const uint32_t sh = 8u * (c & 3u), h = c >> 2;
-        const float tnx = fmaf((float)((nx[h] >> sh) & 0xffu), ax, bx), tfx = fmaf((float)((fx[h] >> sh) & 0xffu), ax, bx);
-        const float tny = fmaf((float)((ny[h] >> sh) & 0xffu), ay, by), tfy = fmaf((float)((fy[h] >> sh) & 0xffu), ay, by);
-        const float tnz = fmaf((float)((nz[h] >> sh) & 0xffu), az, bz), tfz = fmaf((float)((fz[h] >> sh) & 0xffu), az, bz);
+        const uint32_t i = c >> 1;
+        const uint32_t wnx = word_of(nx, i), wfx = word_of(fx, i), wny = word_of(ny, i);
+        const uint32_t wfy = word_of(fy, i), wnz = word_of(nz, i), wfz = word_of(fz, i);
+        const bool hi = (c & 1u) != 0;
+        const float tnx = fmaf(hi ? step_hi(wnx) : step_lo(wnx), ax, bx), tfx = fmaf(hi ? step_hi(wfx) : step_lo(wfx), ax, bx);
+        const float tny = fmaf(hi ? step_hi(wny) : step_lo(wny), ay, by), tfy = fmaf(hi ? step_hi(wfy) : step_lo(wfy), ay, by);
+        const float tnz = fmaf(hi ? step_hi(wnz) : step_lo(wnz), az, bz), tfz = fmaf(hi ? step_hi(wfz) : step_lo(wfz), az, bz);
         float a = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
         float b = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         if (a <= b) mask |= 1u << c;
     }
-    return mask & node_valid(n);
+    return mask & (node_leaves(n) | node_inner(n));
 }
 
 __device__ __forceinline__ float widen(double t) {
@@ -642,16 +657,17 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         } else if (on_node) {
             const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, cur);
             const uint32_t m = node_hits(nd, r, tbf);   // children entering before tmax
-            const uint32_t leaves = nd.c6.z & 0xffu;     // BvhNode::leaf_mask
-            const uint32_t im = m & ~leaves;
-            uint32_t lm = m & leaves;
+            const uint32_t im = m & node_inner(nd);
+            uint32_t lm = m & node_leaves(nd);
 #pragma unroll
             for (int c = 7; c >= 0; --c)
-                if (im & (1u << c)) S.push(sp, node_child(nd, c));
+                if (im & (1u << c)) S.push(sp, node_inner_child(nd, (uint32_t)c));
             while (lm) {
                 const uint32_t c = 31u - (uint32_t)__builtin_clz(lm);
                 lm &= ~(1u << c);
-                S.push(sp, leaf_entry((uint32_t)(~node_child(nd, c)), node_count(nd, c)));
+                uint32_t first, cnt;
+                node_leaf_child(nd, c, first, cnt);
+                S.push(sp, leaf_entry(first, cnt));
             }
             if (sp == 0) live = false;
             else cur = S.pop(sp);
@@ -1506,12 +1522,12 @@ struct Tracer {
                     while (msk) {
                         uint32_t c = (uint32_t)__builtin_ctz(msk);
                         msk &= msk - 1u;
-                        int32_t ch = node_child(nd, c);
-                        if (ch >= 0) {
-                            S.push(sp, ch);
+                        if (node_inner(nd) & (1u << c)) {
+                            S.push(sp, node_inner_child(nd, c));
                             continue;
                         }
-                        uint32_t first = (uint32_t)(~ch), cnt = node_count(nd, c);
+                        uint32_t first, cnt;
+                        node_leaf_child(nd, c, first, cnt);
                         for (uint32_t kk = 0; kk < cnt; ++kk) {
                             double t, u, v;
                             int32_t f;
