@@ -90,7 +90,7 @@ def algorithmic_bytes_per_hit(model, tex_hit_fraction, n_rays=20000, radius=4.0,
     return b, nodes, prims
 
 
-def cpu_baseline(model, run_s=5.0, repeats=3, name="beam-pipe"):
+def cpu_baseline(model, run_s=3.0, repeats=5, name="beam-pipe"):
     """The CPU oracle (one tracer per OpenMP thread, thread-local tallies,
     the reference's CPU parallelism) on a bounded sample of the workload:
     a calibration run of at least a second sizes `repeats` runs of about
