@@ -657,8 +657,24 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
         } else if (on_node) {
             const NodeRegs nd = load_node_at(P.nodes, lnodes, P.lds_nodes, cur);
             const uint32_t m = node_hits(nd, r, tbf);   // children entering before tmax
-            const uint32_t im = m & node_inner(nd);
+            uint32_t im = m & node_inner(nd);
             uint32_t lm = m & node_leaves(nd);
+            // the entry the stack would hand back first (the lowest leaf
+            // slot, else the lowest inner slot) becomes the next one
+            // directly, without its push and pop
+            int32_t next = 0;
+            const bool direct = (im | lm) != 0;
+            if (lm) {
+                const uint32_t c = (uint32_t)__builtin_ctz(lm);
+                lm &= lm - 1u;
+                uint32_t first, cnt;
+                node_leaf_child(nd, c, first, cnt);
+                next = leaf_entry(first, cnt);
+            } else if (im) {
+                const uint32_t c = (uint32_t)__builtin_ctz(im);
+                im &= im - 1u;
+                next = node_inner_child(nd, c);
+            }
 #pragma unroll
             for (int c = 7; c >= 0; --c)
                 if (im & (1u << c)) S.push(sp, node_inner_child(nd, (uint32_t)c));
@@ -669,7 +685,8 @@ __device__ HitRec traverse(const TraceParams& P, int32_t root, d3 Pp, d3 D, int3
                 node_leaf_child(nd, c, first, cnt);
                 S.push(sp, leaf_entry(first, cnt));
             }
-            if (sp == 0) live = false;
+            if (direct) cur = next;
+            else if (sp == 0) live = false;
             else cur = S.pop(sp);
         }
     }
