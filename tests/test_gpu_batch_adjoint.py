@@ -68,6 +68,11 @@ def test_launch_shape_does_not_change_results(sim):
     for other in res[1:]:
         assert res[0][0] == other[0] and np.array_equal(res[0][2], other[2]) and np.array_equal(res[0][4], other[4])
         assert np.allclose(res[0][5], other[5], rtol=1e-9, atol=1e-9)
+    # block sizes other than multiples of 64 up to 256, 768 and 1024 are refused
+    from molflow_sources_amd.simulation import MolflowGpuError
+    for bad in (512, 320, 96 + 1, 2048):
+        with pytest.raises(MolflowGpuError):
+            sim.set_launch(bad, 0)
 
 
 def test_runs_continue_particle_streams(sim):
