@@ -1952,11 +1952,12 @@ namespace {
 typedef void (*TraceKernel)(TraceParams);
 // [feature set][waves per SIMD - 1]; null where not instantiated
 // The one-block-per-CU kernels (lean and low-flux feature sets, 3 or 4
-// waves per SIMD: 768 or 1024 lanes).
+// waves per SIMD: 768 or 1024 lanes; low flux with the adjoint at 3).
 TraceKernel kernel_cu(uint32_t features, int waves) {
     if (waves == 3) {
         if (features == kLean) return k_trace_cu<kLean, 3>;
         if (features == kLowFlux) return k_trace_cu<kLowFlux, 3>;
+        if (features == kLowFluxAdj) return k_trace_cu<kLowFluxAdj, 3>;
     } else if (waves == 4) {
         if (features == kLean) return k_trace_cu<kLean, 4>;
         if (features == kLowFlux) return k_trace_cu<kLowFlux, 4>;

@@ -162,6 +162,27 @@ def test_adjoint_matches_oracle_path_scores(sim):
     assert np.allclose(g, up @ jac / n, rtol=1e-9, atol=1e-12)
 
 
+def test_adjoint_one_block_per_cu_matches_256_lane_blocks(sim):
+    """The low-flux adjoint kernel in one 768-lane block per CU follows the
+    same particles as 256-lane blocks: the same J0 and Jacobian up to the
+    order of the FP64 sums."""
+    m, grp, base, tg = adjoint_model()
+    sim.load(m)
+    sim.rebuild_accel()
+    sim.adjoint_setup(grp, tg)
+    out = []
+    for block in (256, 768):
+        sim.set_launch(block, 0)
+        sim.reset(seed=21)
+        sim.adjoint_set_sticking(base)
+        sim.run(6000, max_events=MAXEV)
+        out.append(sim.adjoint_read())
+    sim.set_launch(256, 0)
+    (a0, aj), (b0, bj) = out
+    assert np.allclose(a0, b0, rtol=1e-9, atol=1e-12)
+    assert np.allclose(aj, bj, rtol=1e-9, atol=1e-12)
+
+
 def test_adjoint_full_paths_statistically(sim):
     """Full low-flux paths: GPU and oracle agree within 4 sigma (batch means)."""
     m, grp, base, tg = adjoint_model()
